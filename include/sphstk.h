@@ -1,0 +1,173 @@
+/*
+ * sphstk.h -- C ABI of the MI355X-native many-sphere Stokes boundary-integral
+ * mobility operator (libsphstk.so).
+ *
+ * Drop-in boundary for the reference's BIE layer (paths relative to the
+ * reference tree):
+ *   SphereSTKSHOperator  BIE/SphereSTKSHOperator.hpp:112-231
+ *   SphereSTKMobMat      BIE/SphereSTKMobMat.hpp:26-79
+ * Tpetra vectors are replaced by plain double arrays (host or device); the
+ * Sphere vector by flat arrays (pos 3N, radius N, quaternion w,x,y,z 4N,
+ * SH order N).  The DOF layout matches the reference: per quadrature point the
+ * interleaved (x,y,z) value, spheres in input order, points in
+ * Shexp::getGridWithPole order with the poles removed (Sphere/Shexp.cpp:206-271,
+ * BIE/SphereSTKSHOperator.cpp:103-121).
+ *
+ * Conventions
+ *  - Every function returns SPHSTK_OK (0) or a negative error code; the message
+ *    of the most recent failure on the calling thread is sphstk_last_error().
+ *    Errors that the reference raises as TEUCHOS_ASSERT / std::logic_error
+ *    (size or mode mismatch, cpp:155-185, MobMat.cpp:100-127) are returned as
+ *    SPHSTK_ERR_INVALID; no C++ exception crosses this boundary.
+ *  - One host thread per handle.  A handle owns one HIP stream on the device
+ *    that was current when it was created; calls on one handle are serialised
+ *    on that stream and are not re-entrant (the reference's scratch members are
+ *    `mutable`, hpp:123-157).
+ *  - Functions whose names end in _device take device pointers that must be
+ *    valid on the handle's device; the others take host pointers.
+ *  - No function falls back to a CPU path: without a usable HIP device,
+ *    creation fails with SPHSTK_ERR_DEVICE.
+ */
+#ifndef SPHSTK_H
+#define SPHSTK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPHSTK_OK 0
+#define SPHSTK_ERR_INVALID (-1)
+#define SPHSTK_ERR_DEVICE (-2)
+#define SPHSTK_ERR_NOMEM (-3)
+#define SPHSTK_ERR_INTERNAL (-4)
+
+/* precision modes of the mobility solve */
+#define SPHSTK_PREC_FP64 0  /* plain form: every operator apply in FP64 */
+#define SPHSTK_PREC_MIXED 1 /* FP32 inner GMRES inside FP64 iterative refinement */
+
+/* creation flags */
+#define SPHSTK_FLAG_NONE 0
+#define SPHSTK_FLAG_TIMING 1 /* record HIP-event timings of the far-field kernels */
+
+typedef struct sphstk_operator sphstk_operator;
+typedef struct sphstk_mobility sphstk_mobility;
+
+/* Belos parameters of mobilitySolver.yaml:3-16 that the solve honours. */
+typedef struct {
+    double tol;         /* Convergence Tolerance, residual scaled by ||b|| (yaml :4, :12-13) */
+    int restart;        /* Num Blocks (yaml :7); 30 */
+    int max_iters;      /* Maximum Iterations (yaml :6) */
+    int explicit_check; /* 1: confirm convergence with ||b - A x|| (Belos explicit residual test) */
+    double inner_tol;   /* mixed precision only: inner FP32 GMRES tolerance (1e-5) */
+    int max_refine;     /* mixed precision only: maximum outer refinement steps */
+} sphstk_solver_params;
+
+typedef struct {
+    int iters;              /* Krylov iterations (operator applies inside GMRES) summed over the solve */
+    int outer;              /* refinement steps (mixed precision), 1 for FP64 */
+    int converged;          /* the reference ignores Belos' status (MobMat.cpp:236); reported here */
+    int shortcut;           /* ||b|| < 1e-5 shortcut taken (MobMat.cpp:216-219) */
+    double relres;          /* final ||b - A x||/||b|| (explicit when computed, else implicit) */
+    int applies_fp64;       /* FP64 operator applies, including the RHS and final single-layer applies */
+    int applies_fp32;       /* FP32 operator applies (mixed precision) */
+} sphstk_solve_info;
+
+/* Last error message of the calling thread ("" if none). */
+const char *sphstk_last_error(void);
+/* Library version string. */
+const char *sphstk_version(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+int sphstk_device_count(void);
+
+/* ---------------------------------------------------------------------------
+ * Operator  Y = beta Y + alpha (cId I + cSL S + cTrac K' + cLOP L) X
+ * Replaces SphereSTKSHOperator(const std::vector<Sphere>*, layerName,
+ *   shared_ptr<STKFMM>&, cId, cSL, cTrac, cLOP)  (SphereSTKSHOperator.hpp:182-184).
+ * Inputs are copied; the grid, near-pair list and tables are built on device. */
+int sphstk_operator_create(int n_spheres, const double *pos, const double *radius, const double *quat_wxyz,
+                           const int *order, double cId, double cSL, double cTrac, double cLOP, int flags,
+                           sphstk_operator **out);
+/* Batched form: n_susp independent suspensions, sphere_counts[b] spheres each,
+ * concatenated in the arrays.  Sphere pairs in different suspensions do not
+ * interact. */
+int sphstk_operator_create_batched(int n_susp, const int *sphere_counts, const double *pos, const double *radius,
+                                   const double *quat_wxyz, const int *order, double cId, double cSL, double cTrac,
+                                   double cLOP, int flags, sphstk_operator **out);
+int sphstk_operator_destroy(sphstk_operator *op);
+
+/* Sizes: spheres, quadrature points, DOF (= 3 * points), near pairs (ordered). */
+int sphstk_operator_sizes(const sphstk_operator *op, int64_t *n_spheres, int64_t *n_points, int64_t *n_dof,
+                          int64_t *n_near_pairs);
+
+/* apply: Y := beta Y + alpha A X for ncol columns of length n_dof, column-major
+ * (SphereSTKSHOperator::apply, cpp:153-236; hpp:195-198). */
+int sphstk_operator_apply(sphstk_operator *op, const double *X, double *Y, int ncol, double alpha, double beta);
+int sphstk_operator_apply_device(sphstk_operator *op, const double *dX, double *dY, int ncol, double alpha,
+                                 double beta);
+
+/* applyP2POP: out += cId in + cSL S[in] + cTrac K'[in]  (cpp:262-485; hpp:204-206). */
+int sphstk_operator_apply_p2p(sphstk_operator *op, const double *in, double *out, double cId, double cSL,
+                              double cTrac);
+int sphstk_operator_apply_p2p_device(sphstk_operator *op, const double *din, double *dout, double cId, double cSL,
+                                     double cTrac);
+
+/* applyLOP: out += cLOP L[in]  (cpp:487-521; hpp:208-210). */
+int sphstk_operator_apply_lop(sphstk_operator *op, const double *in, double *out, double cLOP);
+
+/* projectNullSpace: in-place projection onto vector SH of degree <= p
+ * (cpp:239-260; hpp:200-202). */
+int sphstk_operator_project_nullspace(sphstk_operator *op, double *inout);
+
+/* Geometry getters (getGridWeights / getGridCoords / getGridNorms /
+ * getGridNumberIndex / getGridNumberLength, hpp:218-223).  Any pointer may be
+ * NULL.  index/length are per sphere (n_spheres entries). */
+int sphstk_operator_get_grid(const sphstk_operator *op, double *weights, double *coords, double *norms,
+                             int64_t *index, int64_t *length);
+
+/* Near pairs (target, source) in the order used internally (grouped by target). */
+int sphstk_operator_get_near_pairs(const sphstk_operator *op, int32_t *trg, int32_t *src);
+
+/* Far-field kernel timing collected with SPHSTK_FLAG_TIMING: total milliseconds,
+ * launches, and the algorithmic pair count per launch, for the traction (kind 0)
+ * or single-layer (kind 1) kernel in FP64 (prec 0) or FP32 (prec 1). */
+int sphstk_operator_kernel_stats(const sphstk_operator *op, int kind, int prec, double *total_ms, int64_t *launches,
+                                 double *pairs_per_launch);
+int sphstk_operator_reset_stats(sphstk_operator *op);
+
+/* ---------------------------------------------------------------------------
+ * Mobility  force/torque (6N) -> velocity/angular velocity (6N)
+ * Replaces SphereSTKMobMat(std::vector<Sphere>*, name, shared_ptr<STKFMM>&,
+ *   viscosity) (SphereSTKMobMat.hpp:57-58); the operator it builds is
+ * A = 0.5 I + K' + L (MobMat.cpp:17). */
+int sphstk_mobility_create(int n_spheres, const double *pos, const double *radius, const double *quat_wxyz,
+                           const int *order, double viscosity, const sphstk_solver_params *params, int precision,
+                           int flags, sphstk_mobility **out);
+int sphstk_mobility_create_batched(int n_susp, const int *sphere_counts, const double *pos, const double *radius,
+                                   const double *quat_wxyz, const int *order, double viscosity,
+                                   const sphstk_solver_params *params, int precision, int flags,
+                                   sphstk_mobility **out);
+int sphstk_mobility_destroy(sphstk_mobility *mob);
+/* The operator A owned by the mobility handle (do not destroy). */
+sphstk_operator *sphstk_mobility_operator(sphstk_mobility *mob);
+int sphstk_mobility_set_params(sphstk_mobility *mob, const sphstk_solver_params *params, int precision);
+
+/* solveMob (MobMat.cpp:146-286).  force = [fx,fy,fz,tx,ty,tz] per sphere,
+ * vel = [vx,vy,vz,wx,wy,wz].  The GMRES initial guess is the stored density
+ * unknown x (xLast, MobMat.cpp:222, :245); info may be NULL.  In batched form
+ * every suspension is solved; info aggregates (iters = max over suspensions). */
+int sphstk_mobility_solve(sphstk_mobility *mob, const double *force, double *vel, sphstk_solve_info *info);
+/* apply: Y := beta Y + alpha M X, one mobility solve per column (MobMat.cpp:98-144). */
+int sphstk_mobility_apply(sphstk_mobility *mob, const double *X, double *Y, int ncol, double alpha, double beta);
+
+/* Density access: getDensitySolution (rho = rho0 + x of the last solve, hpp:76),
+ * writeBackDensitySolution semantics (hpp:78) and the warm start x (xLast). */
+int sphstk_mobility_get_density(const sphstk_mobility *mob, double *rho);
+int sphstk_mobility_get_unknown(const sphstk_mobility *mob, double *x);
+int sphstk_mobility_set_unknown(sphstk_mobility *mob, const double *x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -641,6 +641,11 @@ static void all_coefficients(const orc_op *op, const double *in, double *coef) {
  *   SL:       u_t += cSL/(8 pi) sum [f/r + r (r.f)/r^3]
  *   traction: t_t += -cTrac 3/(4 pi) sum r (r.f)(r.n_t)/r^5,  r = x_t - y_s, f = w sigma */
 void orc_far_sums(const orc_op *op, const double *in, double *out, double cSL, double cTrac) {
+    orc_far_sums_range(op, in, out, cSL, cTrac, 0, op->npts);
+}
+
+/* Far sums for the target points [t0, t1) only (all sources). */
+void orc_far_sums_range(const orc_op *op, const double *in, double *out, double cSL, double cTrac, long t0, long t1) {
     const long n = op->npts;
     double *fx = (double *)malloc(sizeof(double) * n), *fy = (double *)malloc(sizeof(double) * n),
            *fz = (double *)malloc(sizeof(double) * n);
@@ -653,7 +658,7 @@ void orc_far_sums(const orc_op *op, const double *in, double *out, double cSL, d
     const double cs = cSL / (8 * PI), ct = -cTrac * 3 / (4 * PI);
     const double *px = op->px, *py = op->py, *pz = op->pz;
 #pragma omp parallel for schedule(dynamic, 64)
-    for (long it = 0; it < n; it++) {
+    for (long it = t0; it < t1; it++) {
         const int T = op->sph[it];
         const double x = px[it], y = py[it], z = pz[it];
         const double nx = op->norms[3 * it], ny = op->norms[3 * it + 1], nz = op->norms[3 * it + 2];

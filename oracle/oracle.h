@@ -81,6 +81,7 @@ void orc_project(const orc_op *op, double *inout);
 
 /* Far-field direct pair sums only (no self/near terms): out += sum over far pairs. */
 void orc_far_sums(const orc_op *op, const double *in, double *out, double cSL, double cTrac);
+void orc_far_sums_range(const orc_op *op, const double *in, double *out, double cSL, double cTrac, long t0, long t1);
 
 typedef struct {
     double tol;

@@ -63,6 +63,8 @@ def lib():
         L.orc_apply.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.c_double]
         L.orc_project.argtypes = [ctypes.c_void_p, _dp]
         L.orc_far_sums.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.c_double]
+        L.orc_far_sums_range.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_long, ctypes.c_long]
         L.orc_gmres.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, GmresParams, ctypes.POINTER(SolveInfo)]
         L.orc_mobility_solve.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp, _dp, _dp, GmresParams,
                                          ctypes.POINTER(SolveInfo)]
@@ -204,6 +206,12 @@ class Operator:
         x = _c(x).ravel()
         out = np.zeros(self.dof)
         lib().orc_far_sums(self._h, _d(x), _d(out), cSL, cTrac)
+        return out
+
+    def far_sums_range(self, x, t0, t1, cSL=0.0, cTrac=0.0):
+        x = _c(x).ravel()
+        out = np.zeros(self.dof)
+        lib().orc_far_sums_range(self._h, _d(x), _d(out), cSL, cTrac, t0, t1)
         return out
 
     def gmres(self, b, x0=None, tol=1e-8, restart=30, max_iters=1000, explicit_check=1):

@@ -1,0 +1,773 @@
+// Host engine: operator set-up (grid, near list, far-field work lists), the
+// operator apply pipeline, batched restarted GMRES and the mobility solve in
+// FP64 and in mixed precision (FP32 far field inside FP64 iterative refinement).
+#include "engine.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <unordered_map>
+
+namespace sphstk {
+
+namespace {
+
+constexpr int kMaxSupportedOrder = 20; // LDS of the expansion kernel bounds the order
+constexpr int64_t kChunk = 16384;
+
+inline void qrot_host(const double *q, const double *v, double *o) {
+    const double w = q[0], ux = q[1], uy = q[2], uz = q[3];
+    const double tx = 2 * (uy * v[2] - uz * v[1]), ty = 2 * (uz * v[0] - ux * v[2]), tz = 2 * (ux * v[1] - uy * v[0]);
+    o[0] = v[0] + w * tx + (uy * tz - uz * ty);
+    o[1] = v[1] + w * ty + (uz * tx - ux * tz);
+    o[2] = v[2] + w * tz + (ux * ty - uy * tx);
+}
+
+using Mask = std::array<uint64_t, kMaxTPL>;
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Operator set-up: SphereSTKSHOperator::setupDOF / setupFMM / setupNearEval
+// (BIE/SphereSTKSHOperator.cpp:37-144, :539-573).
+
+Operator::Operator(int nsusp, const int *counts, const double *pos, const double *radius, const double *quat,
+                   const int *order, double cId, double cSL, double cTrac, double cLOP, int flags)
+    : nsph_(0), nsusp_(nsusp), npts_(0), flags_(flags) {
+    c_[0] = cId;
+    c_[1] = cSL;
+    c_[2] = cTrac;
+    c_[3] = cLOP;
+    SPHSTK_REQUIRE(nsusp >= 1, "number of suspensions must be >= 1");
+    SPHSTK_REQUIRE(counts != nullptr, "sphere counts must not be NULL");
+    susp_sph_off_.assign(nsusp + 1, 0);
+    for (int b = 0; b < nsusp; b++) {
+        SPHSTK_REQUIRE(counts[b] >= 0, "sphere count must be >= 0");
+        susp_sph_off_[b + 1] = susp_sph_off_[b] + counts[b];
+    }
+    nsph_ = (int)susp_sph_off_[nsusp];
+    SPHSTK_REQUIRE(nsph_ == 0 || (pos && radius && quat && order), "input arrays must not be NULL");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) throw Error(-2, "no HIP device available");
+    SPHSTK_HIP(hipGetDevice(&device_));
+
+    pos_.assign(pos, pos + 3 * (size_t)nsph_);
+    rad_.assign(radius, radius + nsph_);
+    order_.assign(order, order + nsph_);
+    quat_.resize(4 * (size_t)nsph_);
+    sph_susp_.resize(nsph_);
+    for (int b = 0; b < nsusp; b++)
+        for (int64_t i = susp_sph_off_[b]; i < susp_sph_off_[b + 1]; i++) sph_susp_[i] = b;
+    for (int i = 0; i < nsph_; i++) {
+        SPHSTK_REQUIRE(std::isfinite(pos_[3 * i]) && std::isfinite(pos_[3 * i + 1]) && std::isfinite(pos_[3 * i + 2]),
+                       "sphere position is not finite (sphere " + std::to_string(i) + ")");
+        SPHSTK_REQUIRE(std::isfinite(rad_[i]) && rad_[i] > 0, "sphere radius must be > 0 (sphere " +
+                                                                   std::to_string(i) + ")");
+        SPHSTK_REQUIRE(order_[i] >= 1 && order_[i] <= kMaxSupportedOrder,
+                       "SH order must be in [1, " + std::to_string(kMaxSupportedOrder) + "] (sphere " +
+                           std::to_string(i) + ")");
+        const double *q = quat + 4 * i;
+        const double nq = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        SPHSTK_REQUIRE(std::isfinite(nq) && nq > 0, "quaternion must be non-zero (sphere " + std::to_string(i) + ")");
+        for (int c = 0; c < 4; c++) quat_[4 * i + c] = q[c] / nq;
+        max_order_ = std::max(max_order_, order_[i]);
+    }
+
+    // DOF layout (setupDOF, cpp:51-80): points per sphere (p+1)(2p+2), spectral 3(p+1)(p+2)
+    pt_off_.assign(nsph_ + 1, 0);
+    coef_off_.assign(nsph_ + 1, 0);
+    rec_off_.assign(nsph_ + 1, 0);
+    for (int i = 0; i < nsph_; i++) {
+        const int p = order_[i];
+        pt_off_[i + 1] = pt_off_[i] + grid_points(p);
+        coef_off_[i + 1] = coef_off_[i] + 3 * (p + 1) * (p + 2);
+        rec_off_[i + 1] = rec_off_[i] + 2 * kRecTrac * tri_size(p + 2);
+    }
+    npts_ = pt_off_[nsph_];
+    coords_.resize(3 * npts_);
+    norms_.resize(3 * npts_);
+    w_.resize(npts_);
+    pt_sph_.resize(npts_);
+    for (int i = 0; i < nsph_; i++) {
+        const ShTables &t = sh_tables(order_[i]);
+        const double R = rad_[i];
+        for (int q = 0; q < t.npts; q++) {
+            const int64_t g = pt_off_[i] + q;
+            double nv[3];
+            qrot_host(&quat_[4 * i], &t.eta[3 * q], nv);
+            for (int c = 0; c < 3; c++) {
+                norms_[3 * g + c] = nv[c];
+                coords_[3 * g + c] = pos_[3 * i + c] + R * nv[c];
+            }
+            w_[g] = R * R * t.w[q];
+            pt_sph_[g] = i;
+        }
+    }
+
+    // near pairs: 0 < |c_s - c_t|^2 < (1.5 R_s + 1.5 R_t)^2 within each suspension
+    // (MPI/NearInteraction.txx:430-441, radiusNear = 1.5 R at cpp:547), grouped
+    // by target and sorted (:527); a cell list replaces the Morton octree.
+    std::vector<std::vector<int>> nb(nsph_);
+    for (int b = 0; b < nsusp; b++) {
+        const int s0 = (int)susp_sph_off_[b], s1 = (int)susp_sph_off_[b + 1];
+        if (s1 <= s0) continue;
+        double rmax = 0, lo[3] = {1e300, 1e300, 1e300};
+        for (int i = s0; i < s1; i++) {
+            rmax = std::max(rmax, rad_[i]);
+            for (int c = 0; c < 3; c++) lo[c] = std::min(lo[c], pos_[3 * i + c]);
+        }
+        const double h = 3.0 * rmax;
+        std::unordered_map<int64_t, std::vector<int>> cells;
+        auto key = [&](int64_t a, int64_t bb, int64_t c) { return (a * 1000003LL + bb) * 1000033LL + c; };
+        std::vector<std::array<int64_t, 3>> cid(s1 - s0);
+        for (int i = s0; i < s1; i++) {
+            for (int c = 0; c < 3; c++) cid[i - s0][c] = (int64_t)std::floor((pos_[3 * i + c] - lo[c]) / h);
+            cells[key(cid[i - s0][0], cid[i - s0][1], cid[i - s0][2])].push_back(i);
+        }
+        for (int t = s0; t < s1; t++) {
+            const auto &ct = cid[t - s0];
+            for (int dx = -1; dx <= 1; dx++)
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dz = -1; dz <= 1; dz++) {
+                        auto it = cells.find(key(ct[0] + dx, ct[1] + dy, ct[2] + dz));
+                        if (it == cells.end()) continue;
+                        for (int s : it->second) {
+                            if (s == t) continue;
+                            const double ddx = pos_[3 * s] - pos_[3 * t], ddy = pos_[3 * s + 1] - pos_[3 * t + 1],
+                                         ddz = pos_[3 * s + 2] - pos_[3 * t + 2];
+                            const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                            const double rs = rad_[s] + rad_[t];
+                            SPHSTK_REQUIRE(d2 >= rs * rs, "overlapping spheres " + std::to_string(t) + " and " +
+                                                              std::to_string(s) + " (centre distance < R_s + R_t)");
+                            const double rr = 1.5 * rad_[s] + 1.5 * rad_[t];
+                            if (d2 > 0 && d2 < rr * rr) nb[t].push_back(s);
+                        }
+                    }
+            std::sort(nb[t].begin(), nb[t].end());
+        }
+    }
+    near_off_.assign(nsph_ + 1, 0);
+    for (int t = 0; t < nsph_; t++) near_off_[t + 1] = near_off_[t] + (int64_t)nb[t].size();
+    near_src_.reserve(near_off_[nsph_]);
+    for (int t = 0; t < nsph_; t++) near_src_.insert(near_src_.end(), nb[t].begin(), nb[t].end());
+
+    // device state
+    SPHSTK_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    d_pos_.upload(pos_);
+    d_rad_.upload(rad_);
+    d_quat_.upload(quat_);
+    d_order_.upload(order_);
+    d_sph_susp_.upload(sph_susp_);
+    d_pt_off_.upload(pt_off_);
+    d_coef_off_.upload(coef_off_);
+    d_rec_off_.upload(rec_off_);
+    d_coords_.upload(coords_);
+    d_norms_.upload(norms_);
+    d_w_.upload(w_);
+    d_near_off_.upload(near_off_);
+    d_near_src_.upload(near_src_);
+    geom_.nsph = nsph_;
+    geom_.nsusp = nsusp_;
+    geom_.npts = npts_;
+    geom_.pos = d_pos_.p;
+    geom_.rad = d_rad_.p;
+    geom_.quat = d_quat_.p;
+    geom_.order = d_order_.p;
+    geom_.sph_susp = d_sph_susp_.p;
+    geom_.pt_off = d_pt_off_.p;
+    geom_.coef_off = d_coef_off_.p;
+    geom_.rec_off = d_rec_off_.p;
+    geom_.coords = d_coords_.p;
+    geom_.norms = d_norms_.p;
+    geom_.w = d_w_.p;
+    geom_.near_off = d_near_off_.p;
+    geom_.near_src = d_near_src_.p;
+    std::vector<bool> used(kMaxOrder + 1, false);
+    for (int p : order_) used[p] = true;
+    for (int p = 0; p <= kMaxOrder; p++)
+        if (used[p]) {
+            d_anal_[p].upload(sh_tables(p).anal);
+            geom_.anal[p] = d_anal_[p].p;
+        }
+
+    build_far_lists(tpl_);
+
+    // segmented-vector chunks (one segment per suspension)
+    std::vector<Chunk> ch;
+    std::vector<int> sco(nsusp_ + 1, 0);
+    for (int b = 0; b < nsusp_; b++) {
+        const int64_t d0 = 3 * pt_off_[susp_sph_off_[b]], d1 = 3 * pt_off_[susp_sph_off_[b + 1]];
+        for (int64_t a = d0; a < d1; a += kChunk) ch.push_back(Chunk{a, std::min(a + kChunk, d1), b, 0});
+        sco[b + 1] = (int)ch.size();
+    }
+    chunks_.upload(ch);
+    seg_chunk_off_.upload(sco);
+    partial_.alloc(std::max<size_t>(ch.size(), 1));
+
+    src64_.alloc(std::max<int64_t>(8 * npts_, 8));
+    coef_.alloc(std::max<int64_t>(coef_off_[nsph_], 1));
+    rec_.alloc(std::max<int64_t>(rec_off_[nsph_], 1));
+    tmp_.alloc(std::max<int64_t>(3 * npts_, 1));
+}
+
+Operator::~Operator() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (auto &e : pending_) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    for (auto &e : free_events_) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// Far-field work lists.  Targets are cut into groups of 64*tpl consecutive points
+// of one suspension.  Each group gets a list of source intervals: the points of
+// its suspension minus, per lane, its own sphere and the spheres near it (the
+// pairs the reference removes again after the FMM, cpp:332-348, :408-433,
+// :641-690).  Groups are launched in order of decreasing work.
+void Operator::build_far_lists(int tpl) {
+    SPHSTK_REQUIRE(tpl == 1 || tpl == 2 || tpl == 4, "targets per lane must be 1, 2 or 4");
+    tpl_ = tpl;
+    const int G = 64 * tpl;
+    std::vector<FarGroup> groups;
+    std::vector<FarInterval> ivs;
+    std::vector<int64_t> work;
+    double pairs = 0.0;
+    Mask full;
+    full.fill(~0ull);
+    std::vector<std::pair<int, Mask>> spec;
+    for (int b = 0; b < nsusp_; b++) {
+        const int s0 = (int)susp_sph_off_[b], s1 = (int)susp_sph_off_[b + 1];
+        const int64_t p0 = pt_off_[s0], p1 = pt_off_[s1];
+        for (int64_t t0 = p0; t0 < p1; t0 += G) {
+            const int cnt = (int)std::min<int64_t>(G, p1 - t0);
+            FarGroup g{t0, cnt, b, (int64_t)ivs.size(), 0};
+            Mask act{};
+            for (int i = 0; i < cnt; i++) act[i / 64] |= 1ull << (i % 64);
+            spec.clear();
+            const int Ta = pt_sph_[t0], Tb = pt_sph_[t0 + cnt - 1];
+            for (int T = Ta; T <= Tb; T++) {
+                const int64_t lo = std::max(pt_off_[T], t0) - t0, hi = std::min(pt_off_[T + 1], t0 + cnt) - t0;
+                Mask lanes{};
+                for (int64_t i = lo; i < hi; i++) lanes[i / 64] |= 1ull << (i % 64);
+                spec.push_back({T, lanes});
+                for (int64_t k = near_off_[T]; k < near_off_[T + 1]; k++) spec.push_back({near_src_[k], lanes});
+            }
+            std::sort(spec.begin(), spec.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+            int64_t gwork = 0;
+            auto emit = [&](int64_t bb, int64_t ee, const Mask &m) {
+                if (bb >= ee) return;
+                if ((int64_t)ivs.size() > g.iv_begin && ivs.back().end == bb &&
+                    std::memcmp(ivs.back().mask, m.data(), sizeof(uint64_t) * kMaxTPL) == 0) {
+                    ivs.back().end = ee;
+                } else {
+                    FarInterval iv;
+                    iv.begin = bb;
+                    iv.end = ee;
+                    std::memcpy(iv.mask, m.data(), sizeof(uint64_t) * kMaxTPL);
+                    ivs.push_back(iv);
+                }
+                int lanes = 0;
+                for (int w = 0; w < tpl; w++) lanes += __builtin_popcountll(m[w] & act[w]);
+                pairs += double(ee - bb) * lanes;
+                gwork += ee - bb;
+            };
+            int cur = s0;
+            size_t k = 0;
+            while (k < spec.size()) {
+                const int S = spec[k].first;
+                Mask ex{};
+                while (k < spec.size() && spec[k].first == S) {
+                    for (int w = 0; w < kMaxTPL; w++) ex[w] |= spec[k].second[w];
+                    k++;
+                }
+                if (S < s0 || S >= s1) continue;
+                emit(pt_off_[cur], pt_off_[S], full);
+                Mask inc;
+                bool any = false;
+                for (int w = 0; w < kMaxTPL; w++) {
+                    inc[w] = (w < tpl) ? ~ex[w] : ~0ull;
+                    if (w < tpl && (inc[w] & act[w])) any = true;
+                }
+                if (any) emit(pt_off_[S], pt_off_[S + 1], inc);
+                cur = S + 1;
+            }
+            emit(pt_off_[cur], pt_off_[s1], full);
+            g.iv_end = (int64_t)ivs.size();
+            groups.push_back(g);
+            work.push_back(gwork);
+        }
+    }
+    // every interval lies inside its group's suspension and every group inside the point range
+    for (const FarGroup &g : groups) {
+        const int b = g.susp;
+        const int64_t lo = pt_off_[susp_sph_off_[b]], hi = pt_off_[susp_sph_off_[b + 1]];
+        SPHSTK_REQUIRE(g.tstart >= lo && g.tstart + g.tcount <= hi && g.tcount > 0 && g.tcount <= G,
+                       "internal: far-field group out of range");
+        for (int64_t k = g.iv_begin; k < g.iv_end; k++)
+            SPHSTK_REQUIRE(ivs[k].begin >= lo && ivs[k].end <= hi && ivs[k].begin < ivs[k].end,
+                           "internal: far-field interval out of range");
+    }
+    std::vector<int> ord(groups.size());
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return work[a] > work[b]; });
+    ngroups_ = (int)groups.size();
+    pairs_ = pairs;
+    groups_.upload(groups);
+    iv_.upload(ivs);
+    gorder_.upload(ord);
+}
+
+void Operator::harvest_events() {
+    for (auto &e : pending_) {
+        SPHSTK_HIP(hipEventSynchronize(e.b));
+        float ms = 0.f;
+        SPHSTK_HIP(hipEventElapsedTime(&ms, e.a, e.b));
+        stats_[e.kind][e.prec].ms += ms;
+        stats_[e.kind][e.prec].launches++;
+        free_events_.push_back(e);
+    }
+    pending_.clear();
+}
+
+void Operator::far(int kind, int prec, double coef, double *out, const int *active) {
+    const bool timing = (flags_ & 1) != 0;
+    EvPair ev{};
+    if (timing) {
+        if (free_events_.empty()) {
+            SPHSTK_HIP(hipEventCreate(&ev.a));
+            SPHSTK_HIP(hipEventCreate(&ev.b));
+        } else {
+            ev = free_events_.back();
+            free_events_.pop_back();
+        }
+        ev.kind = kind;
+        ev.prec = prec;
+        SPHSTK_HIP(hipEventRecord(ev.a, stream_));
+    }
+    const void *src = prec == 0 ? (const void *)src64_.p : (const void *)src32_.p;
+    launch_far(kind, prec, tpl_, geom_, src, trg32_.p, ngroups_, gorder_.p, groups_.p, iv_.p, active, out, coef, 1,
+               stream_);
+    SPHSTK_HIP(hipGetLastError());
+    if (timing) {
+        SPHSTK_HIP(hipEventRecord(ev.b, stream_));
+        pending_.push_back(ev);
+        if (pending_.size() > 64) harvest_events();
+    }
+}
+
+// applyP2POP (cpp:262-485), accumulating into out.
+void Operator::p2p(const double *in, double *out, double cId, double cSL, double cTrac, int prec, const int *active) {
+    const bool doSL = std::fabs(cSL) > 1e-9, doT = std::fabs(cTrac) > 1e-9;
+    double cid = std::fabs(cId) > 1e-9 ? cId : 0.0;
+    if (!doSL && !doT) {
+        if (cid != 0.0) launch_identity(geom_, in, out, cid, active, stream_);
+        return;
+    }
+    if (nsph_ == 0) return;
+    launch_analysis(geom_, in, coef_.p, active, stream_, max_order_);
+    if (prec == 0) {
+        launch_pack_src_f64(geom_, in, src64_.p, stream_);
+    } else {
+        if (!trg32_.p) {
+            trg32_.alloc(8 * npts_);
+            src32_.alloc(8 * npts_);
+            launch_pack_trg_f32(geom_, trg32_.p, stream_);
+        }
+        launch_pack_src_f32(geom_, in, src32_.p, stream_);
+    }
+    if (doT) {
+        far(0, prec, -3.0 / (4.0 * kPi) * cTrac, out, active);
+        launch_expand(geom_, 1, coef_.p, rec_.p, active, stream_, max_order_);
+        launch_nearself(geom_, 1, rec_.p, in, out, 0.0, cTrac, cid, active, stream_);
+        cid = 0.0;
+    }
+    if (doSL) {
+        far(1, prec, cSL / (8.0 * kPi), out, active);
+        launch_expand(geom_, 0, coef_.p, rec_.p, active, stream_, max_order_);
+        launch_nearself(geom_, 0, rec_.p, in, out, cSL, 0.0, cid, active, stream_);
+    }
+    SPHSTK_HIP(hipGetLastError());
+}
+
+void Operator::lop(const double *in, double *out, double cLOP, const int *active) {
+    if (std::fabs(cLOP) > 1e-9) launch_lop(geom_, in, out, cLOP, active, stream_);
+}
+
+void Operator::apply_into(const double *in, double *out, int prec, const int *active) {
+    launch_fill(3 * npts_, 0.0, out, stream_);
+    p2p(in, out, c_[0], c_[1], c_[2], prec, active);
+    lop(in, out, c_[3], active);
+}
+
+// apply (cpp:153-236): Y := beta Y + alpha A X
+void Operator::apply(const double *X, double *Y, double alpha, double beta, int prec) {
+    apply_into(X, tmp_.p, prec, nullptr);
+    launch_axpby(3 * npts_, alpha, tmp_.p, beta, Y, stream_);
+}
+
+// projectNullSpace (cpp:239-260)
+void Operator::project(double *inout) {
+    if (nsph_ == 0) return;
+    launch_analysis(geom_, inout, coef_.p, nullptr, stream_, max_order_);
+    launch_expand(geom_, 1, coef_.p, rec_.p, nullptr, stream_, max_order_);
+    launch_synthesis(geom_, rec_.p, inout, stream_);
+    SPHSTK_HIP(hipGetLastError());
+}
+
+void Operator::grid(double *weights, double *coords, double *norms, int64_t *index, int64_t *length) const {
+    if (weights) std::copy(w_.begin(), w_.end(), weights);
+    if (coords) std::copy(coords_.begin(), coords_.end(), coords);
+    if (norms) std::copy(norms_.begin(), norms_.end(), norms);
+    for (int i = 0; i < nsph_; i++) {
+        if (index) index[i] = pt_off_[i];
+        if (length) length[i] = pt_off_[i + 1] - pt_off_[i];
+    }
+}
+
+void Operator::near_pairs(int32_t *trg, int32_t *src) const {
+    for (int t = 0; t < nsph_; t++)
+        for (int64_t k = near_off_[t]; k < near_off_[t + 1]; k++) {
+            if (trg) trg[k] = t;
+            if (src) src[k] = near_src_[k];
+        }
+}
+
+void Operator::kernel_stats(int kind, int prec, double *ms, int64_t *launches, double *pairs) {
+    SPHSTK_REQUIRE(kind == 0 || kind == 1, "kind must be 0 (traction) or 1 (single layer)");
+    SPHSTK_REQUIRE(prec == 0 || prec == 1, "prec must be 0 (FP64) or 1 (FP32)");
+    harvest_events();
+    if (ms) *ms = stats_[kind][prec].ms;
+    if (launches) *launches = stats_[kind][prec].launches;
+    if (pairs) *pairs = pairs_;
+}
+
+void Operator::reset_stats() {
+    harvest_events();
+    for (auto &a : stats_)
+        for (auto &b : a) b = KernelStats{};
+}
+
+// ---------------------------------------------------------------------------
+// Mobility: SphereSTKMobMat (BIE/SphereSTKMobMat.cpp:6-307) with A = 0.5 I + K' + L (:17)
+
+Mobility::Mobility(int nsusp, const int *counts, const double *pos, const double *radius, const double *quat,
+                   const int *order, double viscosity, const SolverParams &prm, int precision, int flags)
+    : mu_(viscosity), prm_(prm), prec_(precision) {
+    SPHSTK_REQUIRE(std::isfinite(viscosity) && viscosity > 0, "viscosity must be > 0");
+    SPHSTK_REQUIRE(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (mixed)");
+    A_ = std::make_unique<Operator>(nsusp, counts, pos, radius, quat, order, 0.5, 0.0, 1.0, 1.0, flags);
+    const int64_t n = std::max<int64_t>(A_->ndof(), 1);
+    rho_.alloc(n);
+    b_.alloc(n);
+    x_.alloc(n);
+    r_.alloc(n);
+    d_.alloc(n);
+    launch_fill(n, 0.0, x_.p, A_->stream());
+    x_zero_ = true;
+    force_.alloc(std::max(6 * A_->nsph(), 1));
+    vel_.alloc(std::max(6 * A_->nsph(), 1));
+    act_.alloc(A_->nsusp());
+    scal_.alloc(4 * (size_t)A_->nsusp());
+    SPHSTK_HIP(hipStreamSynchronize(A_->stream()));
+}
+
+void Mobility::seg_norms(const double *v, std::vector<double> &out) {
+    Operator &A = *A_;
+    const int B = A.nsusp();
+    launch_seg_dot(v, v, A.chunks(), A.nchunks(), A.scratch_partial(), nullptr, A.stream());
+    launch_seg_reduce(A.scratch_partial(), A.seg_chunk_off(), B, scal_.p, 0, nullptr, A.stream());
+    out.resize(B);
+    SPHSTK_HIP(hipMemcpyAsync(out.data(), scal_.p, sizeof(double) * B, hipMemcpyDeviceToHost, A.stream()));
+    SPHSTK_HIP(hipStreamSynchronize(A.stream()));
+    for (auto &x : out) x = std::sqrt(std::max(x, 0.0));
+}
+
+// Restarted GMRES over all suspensions in lock step (Belos GMRES with Num Blocks =
+// restart, IMGS orthogonalisation = two passes of modified Gram-Schmidt, residual
+// scaled by ||b||; mobilitySolver.yaml:3-16).  Each suspension has its own
+// Krylov space, Hessenberg matrix and Givens rotations; converged suspensions
+// drop out of the operator applies through the device `active` mask.
+void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int prec, std::vector<int> &act,
+                     std::vector<SegInfo> &info, int &applies) {
+    Operator &A = *A_;
+    hipStream_t st = A.stream();
+    const int B = A.nsusp();
+    const int m = std::max(1, prm_.restart);
+    const int64_t n = A.ndof();
+    if (V_.n < size_t(m + 1) * n) V_.alloc(size_t(m + 1) * n);
+    h_.alloc(size_t(2 * (m + 1) + 1) * B);
+    coefv_.alloc(size_t(m) * B);
+    const Chunk *ch = A.chunks();
+    const int nch = A.nchunks();
+    const int *sco = A.seg_chunk_off();
+    double *part = A.scratch_partial();
+
+    std::vector<double> bn, beta;
+    seg_norms(b, bn);
+    for (int s = 0; s < B; s++)
+        if (bn[s] == 0.0 && act[s]) {
+            act[s] = 0;
+            info[s].converged = 1;
+            info[s].relres = 0.0;
+        }
+    auto upload_act = [&](const std::vector<int> &a) {
+        SPHSTK_HIP(hipMemcpyAsync(act_.p, a.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
+    };
+    // r = b - A x
+    if (x_zero) {
+        launch_axpby(n, 1.0, b, 0.0, r_.p, st);
+    } else {
+        upload_act(act);
+        A.apply_into(x, r_.p, prec, act_.p);
+        applies++;
+        launch_axpby(n, 1.0, b, -1.0, r_.p, st);
+    }
+    seg_norms(r_.p, beta);
+    for (int s = 0; s < B; s++)
+        if (act[s]) {
+            info[s].relres = beta[s] / bn[s];
+            if (info[s].relres <= tol) {
+                act[s] = 0;
+                info[s].converged = 1;
+            }
+        }
+    std::vector<double> H(size_t(B) * (m + 1) * m), cs(size_t(B) * m), sn(size_t(B) * m), g(size_t(B) * (m + 1)),
+        hh(size_t(2 * (m + 1) + 1) * B), coef(size_t(m) * B), inv(B);
+    std::vector<int> kdim(B), impok(B), cur(B);
+    int total = 0;
+    auto any = [&](const std::vector<int> &a) { return std::any_of(a.begin(), a.end(), [](int v) { return v != 0; }); };
+    while (any(act) && total < prm_.max_iters) {
+        const std::vector<int> cyc = act;
+        cur = cyc;
+        upload_act(cyc);
+        for (int s = 0; s < B; s++) inv[s] = cyc[s] ? 1.0 / beta[s] : 0.0;
+        SPHSTK_HIP(hipMemcpyAsync(scal_.p, inv.data(), sizeof(double) * B, hipMemcpyHostToDevice, st));
+        launch_seg_scale(V_.p, r_.p, scal_.p, ch, nch, act_.p, st);
+        std::fill(H.begin(), H.end(), 0.0);
+        std::fill(g.begin(), g.end(), 0.0);
+        for (int s = 0; s < B; s++) {
+            g[size_t(s) * (m + 1)] = beta[s];
+            kdim[s] = 0;
+            impok[s] = 0;
+        }
+        for (int j = 0; j < m; j++) {
+            double *W = V_.p + size_t(j + 1) * n;
+            const double *Vj = V_.p + size_t(j) * n;
+            A.apply_into(Vj, W, prec, act_.p);
+            applies++;
+            for (int pass = 0; pass < 2; pass++)
+                for (int i = 0; i <= j; i++) {
+                    double *hv = h_.p + size_t(pass * (m + 1) + i) * B;
+                    launch_seg_dot(W, V_.p + size_t(i) * n, ch, nch, part, act_.p, st);
+                    launch_seg_reduce(part, sco, B, hv, 0, act_.p, st);
+                    launch_seg_axpy(W, V_.p + size_t(i) * n, hv, -1.0, ch, nch, act_.p, st);
+                }
+            double *nv = h_.p + size_t(2 * (m + 1)) * B;
+            launch_seg_dot(W, W, ch, nch, part, act_.p, st);
+            launch_seg_reduce(part, sco, B, nv, 0, act_.p, st);
+            launch_seg_reduce(part, sco, B, scal_.p, 1, act_.p, st);
+            launch_seg_scale(W, W, scal_.p, ch, nch, act_.p, st);
+            SPHSTK_HIP(hipMemcpyAsync(hh.data(), h_.p, sizeof(double) * hh.size(), hipMemcpyDeviceToHost, st));
+            SPHSTK_HIP(hipStreamSynchronize(st));
+            for (int s = 0; s < B; s++) {
+                if (!cur[s]) continue;
+                double *Hs = H.data() + size_t(s) * (m + 1) * m;
+                double *cs_ = cs.data() + size_t(s) * m, *sn_ = sn.data() + size_t(s) * m;
+                double *gs = g.data() + size_t(s) * (m + 1);
+                for (int i = 0; i <= j; i++)
+                    Hs[i * m + j] = hh[size_t(i) * B + s] + hh[size_t((m + 1) + i) * B + s];
+                const double hn = std::sqrt(std::max(hh[size_t(2 * (m + 1)) * B + s], 0.0));
+                Hs[(j + 1) * m + j] = hn;
+                for (int i = 0; i < j; i++) {
+                    const double t0 = cs_[i] * Hs[i * m + j] + sn_[i] * Hs[(i + 1) * m + j];
+                    const double t1 = -sn_[i] * Hs[i * m + j] + cs_[i] * Hs[(i + 1) * m + j];
+                    Hs[i * m + j] = t0;
+                    Hs[(i + 1) * m + j] = t1;
+                }
+                const double a = Hs[j * m + j], bb = Hs[(j + 1) * m + j];
+                const double den = std::sqrt(a * a + bb * bb);
+                cs_[j] = den > 0 ? a / den : 1.0;
+                sn_[j] = den > 0 ? bb / den : 0.0;
+                Hs[j * m + j] = den;
+                Hs[(j + 1) * m + j] = 0.0;
+                gs[j + 1] = -sn_[j] * gs[j];
+                gs[j] = cs_[j] * gs[j];
+                kdim[s] = j + 1;
+                info[s].iters++;
+                info[s].relres = std::fabs(gs[j + 1]) / bn[s];
+                if (info[s].relres <= tol) impok[s] = 1;
+                if (info[s].relres <= tol || hn == 0.0) cur[s] = 0;
+            }
+            total++;
+            if (!any(cur) || total >= prm_.max_iters) break;
+            upload_act(cur);
+        }
+        // x += V y (back substitution of the rotated Hessenberg system)
+        std::fill(coef.begin(), coef.end(), 0.0);
+        int kmax = 0;
+        for (int s = 0; s < B; s++) {
+            if (!cyc[s]) continue;
+            const double *Hs = H.data() + size_t(s) * (m + 1) * m;
+            const double *gs = g.data() + size_t(s) * (m + 1);
+            std::vector<double> y(kdim[s]);
+            for (int i = kdim[s] - 1; i >= 0; i--) {
+                double acc = gs[i];
+                for (int k = i + 1; k < kdim[s]; k++) acc -= Hs[i * m + k] * y[k];
+                y[i] = Hs[i * m + i] != 0.0 ? acc / Hs[i * m + i] : 0.0;
+            }
+            for (int i = 0; i < kdim[s]; i++) coef[size_t(i) * B + s] = y[i];
+            kmax = std::max(kmax, kdim[s]);
+        }
+        upload_act(cyc);
+        SPHSTK_HIP(hipMemcpyAsync(coefv_.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, st));
+        launch_seg_multi_axpy(x, V_.p, n, coefv_.p, kmax, B, ch, nch, act_.p, st);
+        bool need_res = prm_.explicit_check != 0;
+        for (int s = 0; s < B; s++)
+            if (cyc[s] && !impok[s]) need_res = true;
+        if (need_res) {
+            A.apply_into(x, r_.p, prec, act_.p);
+            applies++;
+            launch_axpby(n, 1.0, b, -1.0, r_.p, st);
+            seg_norms(r_.p, beta);
+        }
+        for (int s = 0; s < B; s++) {
+            if (!cyc[s]) continue;
+            bool conv = impok[s] != 0;
+            if (need_res) {
+                info[s].relres = beta[s] / bn[s];
+                if (prm_.explicit_check) conv = conv && info[s].relres <= tol;
+            }
+            if (conv) {
+                info[s].converged = 1;
+                act[s] = 0;
+            }
+        }
+    }
+}
+
+// solveMob (BIE/SphereSTKMobMat.cpp:146-286)
+void Mobility::solve(const double *force, double *vel, std::vector<SegInfo> &info, int &outer, int &app64,
+                     int &app32) {
+    Operator &A = *A_;
+    hipStream_t st = A.stream();
+    const int B = A.nsusp();
+    const int N = A.nsph();
+    const int64_t n = A.ndof();
+    info.assign(B, SegInfo{});
+    outer = 0;
+    app64 = app32 = 0;
+    if (N == 0) return;
+    SPHSTK_HIP(hipMemcpyAsync(force_.p, force, sizeof(double) * 6 * N, hipMemcpyHostToDevice, st));
+    launch_rho0(A.geom(), force_.p, rho_.p, st);
+    // b = -(1/2 I + K') rho0   (:196)
+    launch_fill(n, 0.0, b_.p, st);
+    A.p2p(rho_.p, b_.p, -0.5, 0.0, -1.0, 0, nullptr);
+    app64++;
+    std::vector<double> bn;
+    seg_norms(b_.p, bn);
+    std::vector<int> act(B, 0);
+    std::vector<double> keep(B, 1.0);
+    const std::vector<int64_t> &so = A.susp_sph_off();
+    for (int s = 0; s < B; s++) {
+        if (so[s + 1] == so[s]) {
+            info[s].converged = 1;
+            continue;
+        }
+        if (bn[s] < 1e-5) { // (:216-219) forcing ~ zero: x = 0
+            info[s].shortcut = 1;
+            info[s].converged = 1;
+            keep[s] = 0.0;
+        } else {
+            act[s] = 1;
+        }
+    }
+    if (!x_zero_) {
+        SPHSTK_HIP(hipMemcpyAsync(scal_.p + B, keep.data(), sizeof(double) * B, hipMemcpyHostToDevice, st));
+        launch_seg_scale(x_.p, x_.p, scal_.p + B, A.chunks(), A.nchunks(), nullptr, st);
+    }
+    if (prec_ == 0) {
+        gmres(b_.p, x_.p, x_zero_, prm_.tol, 0, act, info, app64);
+        outer = 1;
+    } else {
+        // iterative refinement: r = b - A64 x; inner FP32-far-field GMRES on A d = r; x += d
+        std::vector<double> rn;
+        if (x_zero_) {
+            launch_axpby(n, 1.0, b_.p, 0.0, r_.p, st);
+        } else {
+            A.apply_into(x_.p, r_.p, 0, nullptr);
+            app64++;
+            launch_axpby(n, 1.0, b_.p, -1.0, r_.p, st);
+        }
+        seg_norms(r_.p, rn);
+        for (int s = 0; s < B; s++)
+            if (act[s]) info[s].relres = rn[s] / bn[s];
+        while (outer < prm_.max_refine) {
+            std::vector<int> a2(B, 0);
+            bool anyact = false;
+            for (int s = 0; s < B; s++) {
+                if (act[s] && info[s].relres <= prm_.tol) {
+                    info[s].converged = 1;
+                    act[s] = 0;
+                }
+                a2[s] = act[s];
+                anyact = anyact || act[s];
+            }
+            if (!anyact) break;
+            std::vector<SegInfo> inner(B);
+            launch_fill(n, 0.0, d_.p, st);
+            std::vector<int> ia = a2;
+            const double itol = std::max(prm_.inner_tol, prm_.tol);
+            // the inner solve overwrites r_, so its right-hand side is a copy of the outer residual
+            gmres_inner_rhs_.alloc(n);
+            launch_axpby(n, 1.0, r_.p, 0.0, gmres_inner_rhs_.p, st);
+            gmres(gmres_inner_rhs_.p, d_.p, true, itol, 1, ia, inner, app32);
+            SPHSTK_HIP(hipMemcpyAsync(act_.p, a2.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
+            std::vector<double> ones(B, 1.0);
+            SPHSTK_HIP(hipMemcpyAsync(scal_.p + 2 * B, ones.data(), sizeof(double) * B, hipMemcpyHostToDevice, st));
+            launch_seg_axpy(x_.p, d_.p, scal_.p + 2 * B, 1.0, A.chunks(), A.nchunks(), act_.p, st);
+            A.apply_into(x_.p, r_.p, 0, act_.p);
+            app64++;
+            launch_axpby(n, 1.0, b_.p, -1.0, r_.p, st);
+            seg_norms(r_.p, rn);
+            for (int s = 0; s < B; s++)
+                if (a2[s]) {
+                    info[s].iters += inner[s].iters;
+                    info[s].relres = rn[s] / bn[s];
+                }
+            outer++;
+        }
+        for (int s = 0; s < B; s++)
+            if (act[s] && info[s].relres <= prm_.tol) info[s].converged = 1;
+    }
+    x_zero_ = false;
+    // rho = rho0 + x; u = S[rho]; rigid-body averages / mu   (:253-285)
+    launch_axpby(n, 1.0, x_.p, 1.0, rho_.p, st);
+    launch_fill(n, 0.0, b_.p, st);
+    A.p2p(rho_.p, b_.p, 0.0, 1.0, 0.0, 0, nullptr);
+    app64++;
+    launch_rigid_avg(A.geom(), b_.p, 1.0 / mu_, vel_.p, st);
+    SPHSTK_HIP(hipMemcpyAsync(vel, vel_.p, sizeof(double) * 6 * N, hipMemcpyDeviceToHost, st));
+    SPHSTK_HIP(hipStreamSynchronize(st));
+}
+
+void Mobility::get_density(double *rho) const {
+    SPHSTK_HIP(hipStreamSynchronize(A_->stream()));
+    SPHSTK_HIP(hipMemcpy(rho, rho_.p, sizeof(double) * A_->ndof(), hipMemcpyDeviceToHost));
+}
+void Mobility::get_unknown(double *x) const {
+    SPHSTK_HIP(hipStreamSynchronize(A_->stream()));
+    SPHSTK_HIP(hipMemcpy(x, x_.p, sizeof(double) * A_->ndof(), hipMemcpyDeviceToHost));
+}
+void Mobility::set_unknown(const double *x) {
+    SPHSTK_HIP(hipStreamSynchronize(A_->stream()));
+    x_zero_ = std::all_of(x, x + A_->ndof(), [](double v) { return v == 0.0; });
+    SPHSTK_HIP(hipMemcpy(x_.p, x, sizeof(double) * A_->ndof(), hipMemcpyHostToDevice));
+}
+
+} // namespace sphstk

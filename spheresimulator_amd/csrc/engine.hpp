@@ -1,0 +1,181 @@
+// Host engine of libsphstk.so: the MI355X-native counterparts of
+// SphereSTKSHOperator (BIE/SphereSTKSHOperator.hpp:112-231) and SphereSTKMobMat
+// (BIE/SphereSTKMobMat.hpp:26-79), over a batch of independent suspensions
+// (a single suspension is a batch of one).
+#pragma once
+
+#include "kernels.hpp"
+
+#include <memory>
+
+namespace sphstk {
+
+// Owning device buffer.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count == n && p) return;
+        release();
+        n = count;
+        if (count) SPHSTK_HIP(hipMalloc(&p, sizeof(T) * count));
+    }
+    void upload(const std::vector<T> &v) {
+        alloc(v.size());
+        if (!v.empty()) SPHSTK_HIP(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    }
+};
+
+struct KernelStats {
+    double ms = 0.0;
+    int64_t launches = 0;
+};
+
+struct SolverParams {
+    double tol = 1e-8;
+    int restart = 30;
+    int max_iters = 1000;
+    int explicit_check = 1;
+    double inner_tol = 1e-5;
+    int max_refine = 10;
+};
+
+struct SegInfo {
+    int iters = 0, converged = 0, shortcut = 0;
+    double relres = 0.0;
+};
+
+class Operator {
+  public:
+    Operator(int nsusp, const int *counts, const double *pos, const double *radius, const double *quat,
+             const int *order, double cId, double cSL, double cTrac, double cLOP, int flags);
+    ~Operator();
+
+    // sizes
+    int nsph() const { return nsph_; }
+    int nsusp() const { return nsusp_; }
+    int64_t npts() const { return npts_; }
+    int64_t ndof() const { return 3 * npts_; }
+    int64_t nnear() const { return (int64_t)near_src_.size(); }
+    double coeff(int i) const { return c_[i]; }
+    hipStream_t stream() const { return stream_; }
+    const DevGeom &geom() const { return geom_; }
+
+    // device-pointer operations (all on stream()); `active` is a device array of
+    // one int per suspension or nullptr for all.
+    // out += cId in + cSL S[in] + cTrac K'[in]   (far field in FP64 (prec 0) or FP32 (prec 1))
+    void p2p(const double *in, double *out, double cId, double cSL, double cTrac, int prec, const int *active);
+    // out += cLOP L[in]
+    void lop(const double *in, double *out, double cLOP, const int *active);
+    // out = A in with the stored coefficients (overwrites out)
+    void apply_into(const double *in, double *out, int prec, const int *active);
+    // Y = beta Y + alpha A X
+    void apply(const double *X, double *Y, double alpha, double beta, int prec);
+    void project(double *inout);
+
+    // host accessors
+    void grid(double *weights, double *coords, double *norms, int64_t *index, int64_t *length) const;
+    void near_pairs(int32_t *trg, int32_t *src) const;
+    void kernel_stats(int kind, int prec, double *ms, int64_t *launches, double *pairs);
+    void reset_stats();
+
+    // segmented-vector helpers (one segment per suspension)
+    const Chunk *chunks() const { return chunks_.p; }
+    int nchunks() const { return (int)chunks_.n; }
+    const int *seg_chunk_off() const { return seg_chunk_off_.p; }
+    const std::vector<int64_t> &susp_sph_off() const { return susp_sph_off_; }
+    const std::vector<int64_t> &pt_off() const { return pt_off_; }
+    double *scratch_partial() { return partial_.p; }
+
+  private:
+    void build_far_lists(int tpl);
+    void far(int kind, int prec, double coef, double *out, const int *active);
+
+    int nsph_, nsusp_;
+    int64_t npts_;
+    double c_[4];
+    int flags_;
+    int tpl_ = 2;
+    int max_order_ = 0;
+    hipStream_t stream_ = nullptr;
+    int device_ = 0;
+
+    std::vector<double> pos_, rad_, quat_;
+    std::vector<int> order_, sph_susp_;
+    std::vector<int64_t> susp_sph_off_, pt_off_, coef_off_, rec_off_, near_off_;
+    std::vector<int> near_src_;
+    std::vector<double> coords_, norms_, w_;
+    std::vector<int32_t> pt_sph_;
+
+    DevBuf<double> d_pos_, d_rad_, d_quat_, d_coords_, d_norms_, d_w_;
+    DevBuf<int> d_order_, d_sph_susp_, d_near_src_;
+    DevBuf<int64_t> d_pt_off_, d_coef_off_, d_rec_off_, d_near_off_;
+    DevBuf<double> d_anal_[kMaxOrder + 1];
+    DevGeom geom_;
+
+    // far-field work lists
+    DevBuf<FarGroup> groups_;
+    DevBuf<FarInterval> iv_;
+    DevBuf<int> gorder_;
+    int ngroups_ = 0;
+    double pairs_ = 0.0; // algorithmic far pairs per launch
+
+    // per-apply scratch
+    DevBuf<double> src64_, coef_, rec_, tmp_;
+    DevBuf<float> src32_, trg32_;
+    DevBuf<Chunk> chunks_;
+    DevBuf<int> seg_chunk_off_;
+    DevBuf<double> partial_;
+
+    // timing
+    struct EvPair {
+        hipEvent_t a, b;
+        int kind, prec;
+    };
+    std::vector<EvPair> pending_;
+    std::vector<EvPair> free_events_;
+    KernelStats stats_[2][2];
+    void harvest_events();
+};
+
+class Mobility {
+  public:
+    Mobility(int nsusp, const int *counts, const double *pos, const double *radius, const double *quat,
+             const int *order, double viscosity, const SolverParams &prm, int precision, int flags);
+    Operator &op() { return *A_; }
+    void set_params(const SolverParams &prm, int precision) {
+        prm_ = prm;
+        prec_ = precision;
+    }
+    // host arrays: force/vel 6N
+    void solve(const double *force, double *vel, std::vector<SegInfo> &info, int &outer, int &app64, int &app32);
+    void get_density(double *rho) const;
+    void get_unknown(double *x) const;
+    void set_unknown(const double *x);
+
+  private:
+    // GMRES on the segments flagged in `act` (host, one per suspension); x and b device
+    void gmres(const double *b, double *x, bool x_zero, double tol, int prec, std::vector<int> &act,
+               std::vector<SegInfo> &info, int &applies);
+    void seg_norms(const double *v, std::vector<double> &out);
+
+    std::unique_ptr<Operator> A_;
+    double mu_;
+    SolverParams prm_;
+    int prec_;
+    bool x_zero_ = true; // the stored unknown is known to be zero (cold start)
+    DevBuf<double> rho_, b_, x_, r_, d_, V_, h_, scal_, coefv_, force_, vel_, gmres_inner_rhs_;
+    DevBuf<int> act_;
+};
+
+} // namespace sphstk

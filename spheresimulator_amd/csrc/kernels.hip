@@ -1,0 +1,921 @@
+// HIP kernels of the many-sphere Stokes boundary-integral operator (gfx950).
+//
+// Far field   k_far_*        direct Stokeslet / stresslet pair sums over the
+//                            far pairs of every target (replaces the STKFMM
+//                            evaluation, BIE/SphereSTKSHOperator.cpp:287-303,
+//                            :365-446).  One 64-lane wave per group of 64*TPL
+//                            targets; sources are wave-uniform and reach the
+//                            VALU through scalar loads (no LDS staging).
+// SH analysis k_analysis     grid values -> vector-SH coefficients per sphere
+//                            (Shexp::calcSpectralCoeff, Sphere/Shexp.cpp:408-447).
+// Expansion   k_expand       coefficients -> exterior solid-harmonic expansion
+//                            records (the flow of SCTL StokesEvalSL/KL/KSelf).
+// Near/self   k_nearself     self (SphereSTKSHOperator.cpp:305-351, :375-456)
+//                            and near-pair (:575-695) SH evaluations.
+// LOP         k_lop          rigid-body projector (:487-521).
+// BLAS-1      k_seg_*        segmented (one segment per suspension) GMRES
+//                            vector kernels.
+#include "kernels.hpp"
+
+namespace sphstk {
+
+// ---------------------------------------------------------------------------
+// helpers
+
+__device__ __forceinline__ void qrot(const double *q, const double v[3], double o[3], bool inv) {
+    const double w = q[0];
+    const double s = inv ? -1.0 : 1.0;
+    const double ux = s * q[1], uy = s * q[2], uz = s * q[3];
+    const double tx = 2.0 * (uy * v[2] - uz * v[1]);
+    const double ty = 2.0 * (uz * v[0] - ux * v[2]);
+    const double tz = 2.0 * (ux * v[1] - uy * v[0]);
+    o[0] = v[0] + w * tx + (uy * tz - uz * ty);
+    o[1] = v[1] + w * ty + (uz * tx - ux * tz);
+    o[2] = v[2] + w * tz + (ux * ty - uy * tx);
+}
+
+// FP64 reciprocal square root: hardware estimate refined by one Newton step,
+// y1 = y0 + y0 (1 - r2 y0^2)/2.
+__device__ __forceinline__ double rsqrt_f64(double r2) {
+    const double y0 = __builtin_amdgcn_rsq(r2);
+    const double h = r2 * y0;
+    const double e = __builtin_fma(-h, y0, 1.0);
+    return __builtin_fma(0.5 * y0, e, y0);
+}
+
+__device__ __forceinline__ double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+__device__ __forceinline__ int dtri(int n, int m) { return n * (n + 1) / 2 + m; }
+
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    T r = 0;
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; i++) r += sh[i];
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// source / target packing
+
+__global__ void k_pack_src_f64(int64_t n, const double *__restrict__ coords, const double *__restrict__ w,
+                               const double *__restrict__ in, double *__restrict__ src) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double wi = w[i];
+    double *o = src + 8 * i;
+    o[0] = coords[3 * i];
+    o[1] = coords[3 * i + 1];
+    o[2] = coords[3 * i + 2];
+    o[3] = wi * in[3 * i];
+    o[4] = wi * in[3 * i + 1];
+    o[5] = wi * in[3 * i + 2];
+    o[6] = 0.0;
+    o[7] = 0.0;
+}
+
+__global__ void k_pack_src_f32(int64_t n, const double *__restrict__ coords, const double *__restrict__ w,
+                               const double *__restrict__ in, float *__restrict__ src) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double wi = w[i];
+    float *o = src + 8 * i;
+    o[0] = (float)coords[3 * i];
+    o[1] = (float)coords[3 * i + 1];
+    o[2] = (float)coords[3 * i + 2];
+    o[3] = (float)(wi * in[3 * i]);
+    o[4] = (float)(wi * in[3 * i + 1]);
+    o[5] = (float)(wi * in[3 * i + 2]);
+    o[6] = 0.f;
+    o[7] = 0.f;
+}
+
+__global__ void k_pack_trg_f32(int64_t n, const double *__restrict__ coords, const double *__restrict__ norms,
+                               float *__restrict__ trg) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float *o = trg + 8 * i;
+    for (int c = 0; c < 3; c++) {
+        o[c] = (float)coords[3 * i + c];
+        o[3 + c] = (float)norms[3 * i + c];
+    }
+    o[6] = 0.f;
+    o[7] = 0.f;
+}
+
+void launch_pack_src_f64(const DevGeom &g, const double *in, double *src, hipStream_t s) {
+    const int64_t n = g.npts;
+    if (!n) return;
+    k_pack_src_f64<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(n, g.coords, g.w, in, src);
+}
+void launch_pack_src_f32(const DevGeom &g, const double *in, float *src, hipStream_t s) {
+    const int64_t n = g.npts;
+    if (!n) return;
+    k_pack_src_f32<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(n, g.coords, g.w, in, src);
+}
+void launch_pack_trg_f32(const DevGeom &g, float *trg, hipStream_t s) {
+    const int64_t n = g.npts;
+    if (!n) return;
+    k_pack_trg_f32<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(n, g.coords, g.norms, trg);
+}
+
+// ---------------------------------------------------------------------------
+// far field, FP64.  KIND 0: traction  t = -(3/4pi) sum r (r.f)(r.n_t)/r^5
+//                   KIND 1: single layer u = (1/8pi) sum [f/r + r (r.f)/r^3]
+// (the -(3/4pi) and 1/(8pi) prefactors are folded into `coef`).
+
+// One source against TPL targets.  MASKED: per-lane inclusion flags select the
+// reciprocal distance (a select, so an excluded coincident point cannot turn
+// into inf*0).
+template <int KIND, int TPL, bool MASKED, typename R>
+__device__ __forceinline__ void far_pair(const R sx, const R sy, const R sz, const R fx, const R fy, const R fz,
+                                         const R *x, const R *y, const R *z, const R *nx, const R *ny, const R *nz,
+                                         const bool *incl, R *ax, R *ay, R *az) {
+#pragma unroll
+    for (int t = 0; t < TPL; t++) {
+        const R dx = x[t] - sx, dy = y[t] - sy, dz = z[t] - sz;
+        const R r2 = fmad(dx, dx, fmad(dy, dy, dz * dz));
+        R ri;
+        if constexpr (sizeof(R) == 8)
+            ri = rsqrt_f64(r2);
+        else
+            ri = __builtin_amdgcn_rsqf(r2);
+        if (MASKED) ri = incl[t] ? ri : R(0);
+        const R df = fmad(dx, fx, fmad(dy, fy, dz * fz));
+        if (KIND == 0) {
+            const R dn = fmad(dx, nx[t], fmad(dy, ny[t], dz * nz[t]));
+            const R ri2 = ri * ri;
+            const R q = (df * dn) * (ri2 * ri2 * ri);
+            ax[t] = fmad(q, dx, ax[t]);
+            ay[t] = fmad(q, dy, ay[t]);
+            az[t] = fmad(q, dz, az[t]);
+        } else {
+            const R q = df * (ri * ri * ri);
+            ax[t] = fmad(q, dx, fmad(fx, ri, ax[t]));
+            ay[t] = fmad(q, dy, fmad(fy, ri, ay[t]));
+            az[t] = fmad(q, dz, fmad(fz, ri, az[t]));
+        }
+    }
+}
+
+// Sources [b, e) of an 8-word record array, four at a time so that the
+// wave-uniform (scalar) loads of a batch are issued before its arithmetic.
+template <int KIND, int TPL, bool MASKED, typename R>
+__device__ __forceinline__ void far_range(const R *__restrict__ src, int64_t b, int64_t e, const R *x, const R *y,
+                                          const R *z, const R *nx, const R *ny, const R *nz, const bool *incl, R *ax,
+                                          R *ay, R *az) {
+    int64_t s = b;
+    const int64_t e4 = b + ((e - b) & ~int64_t(3));
+    for (; s != e4; s += 4) {
+        R v[4][6];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int c = 0; c < 6; c++) v[u][c] = src[8 * (s + u) + c];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            far_pair<KIND, TPL, MASKED>(v[u][0], v[u][1], v[u][2], v[u][3], v[u][4], v[u][5], x, y, z, nx, ny, nz,
+                                        incl, ax, ay, az);
+    }
+    for (; s != e; s++) {
+        const R *sp = src + 8 * s;
+        far_pair<KIND, TPL, MASKED>(sp[0], sp[1], sp[2], sp[3], sp[4], sp[5], x, y, z, nx, ny, nz, incl, ax, ay, az);
+    }
+}
+
+template <int TPL>
+__device__ __forceinline__ void store_far(const FarGroup &G, int lane, const double *ax, const double *ay,
+                                          const double *az, double *__restrict__ out, double coef, int accumulate) {
+#pragma unroll
+    for (int t = 0; t < TPL; t++) {
+        const int idx = t * 64 + lane;
+        if (idx < G.tcount) {
+            double *o = out + 3 * (G.tstart + idx);
+            if (accumulate) {
+                o[0] += coef * ax[t];
+                o[1] += coef * ay[t];
+                o[2] += coef * az[t];
+            } else {
+                o[0] = coef * ax[t];
+                o[1] = coef * ay[t];
+                o[2] = coef * az[t];
+            }
+        }
+    }
+}
+
+template <int KIND, int TPL>
+__global__ __launch_bounds__(64) void k_far_f64(const double *__restrict__ src, const double *__restrict__ coords,
+                                                const double *__restrict__ norms, const int *__restrict__ gorder,
+                                                const FarGroup *__restrict__ groups,
+                                                const FarInterval *__restrict__ iv, const int *__restrict__ active,
+                                                double *__restrict__ out, double coef, int accumulate) {
+    const int gi = gorder[blockIdx.x];
+    const FarGroup G = groups[gi];
+    if (active && !active[G.susp]) return;
+    const int lane = threadIdx.x;
+    double x[TPL], y[TPL], z[TPL], nx[TPL], ny[TPL], nz[TPL], ax[TPL], ay[TPL], az[TPL];
+#pragma unroll
+    for (int t = 0; t < TPL; t++) {
+        const int idx = t * 64 + lane;
+        const int64_t tid = G.tstart + (idx < G.tcount ? idx : 0);
+        x[t] = coords[3 * tid];
+        y[t] = coords[3 * tid + 1];
+        z[t] = coords[3 * tid + 2];
+        nx[t] = norms[3 * tid];
+        ny[t] = norms[3 * tid + 1];
+        nz[t] = norms[3 * tid + 2];
+        ax[t] = ay[t] = az[t] = 0.0;
+    }
+    for (int64_t k = G.iv_begin; k < G.iv_end; k++) {
+        const int64_t b = iv[k].begin, e = iv[k].end;
+        bool full = true, incl[TPL];
+#pragma unroll
+        for (int t = 0; t < TPL; t++) {
+            const uint64_t m = iv[k].mask[t];
+            full = full && (m == ~0ull);
+            incl[t] = ((m >> lane) & 1ull) != 0;
+        }
+        if (full)
+            far_range<KIND, TPL, false>(src, b, e, x, y, z, nx, ny, nz, incl, ax, ay, az);
+        else
+            far_range<KIND, TPL, true>(src, b, e, x, y, z, nx, ny, nz, incl, ax, ay, az);
+    }
+    store_far<TPL>(G, lane, ax, ay, az, out, coef, accumulate);
+}
+
+// far field, FP32 pair arithmetic with two-level accumulation: FP32 partial sums
+// over blocks of at most kF32Block sources, flushed into FP64 accumulators.
+constexpr int kF32Block = 256;
+
+template <int KIND, int TPL>
+__global__ __launch_bounds__(64) void k_far_f32(const float *__restrict__ src, const float *__restrict__ trg,
+                                                const int *__restrict__ gorder, const FarGroup *__restrict__ groups,
+                                                const FarInterval *__restrict__ iv, const int *__restrict__ active,
+                                                double *__restrict__ out, double coef, int accumulate) {
+    const int gi = gorder[blockIdx.x];
+    const FarGroup G = groups[gi];
+    if (active && !active[G.susp]) return;
+    const int lane = threadIdx.x;
+    float x[TPL], y[TPL], z[TPL], nx[TPL], ny[TPL], nz[TPL];
+    double ax[TPL], ay[TPL], az[TPL];
+#pragma unroll
+    for (int t = 0; t < TPL; t++) {
+        const int idx = t * 64 + lane;
+        const int64_t tid = G.tstart + (idx < G.tcount ? idx : 0);
+        const float *tp = trg + 8 * tid;
+        x[t] = tp[0];
+        y[t] = tp[1];
+        z[t] = tp[2];
+        nx[t] = tp[3];
+        ny[t] = tp[4];
+        nz[t] = tp[5];
+        ax[t] = ay[t] = az[t] = 0.0;
+    }
+    for (int64_t k = G.iv_begin; k < G.iv_end; k++) {
+        const int64_t b = iv[k].begin, e = iv[k].end;
+        bool full = true, incl[TPL];
+#pragma unroll
+        for (int t = 0; t < TPL; t++) {
+            const uint64_t m = iv[k].mask[t];
+            full = full && (m == ~0ull);
+            incl[t] = ((m >> lane) & 1ull) != 0;
+        }
+        for (int64_t s0 = b; s0 < e; s0 += kF32Block) {
+            const int64_t s1 = (s0 + kF32Block < e) ? s0 + kF32Block : e;
+            float px[TPL], py[TPL], pz[TPL];
+#pragma unroll
+            for (int t = 0; t < TPL; t++) px[t] = py[t] = pz[t] = 0.f;
+            if (full)
+                far_range<KIND, TPL, false>(src, s0, s1, x, y, z, nx, ny, nz, incl, px, py, pz);
+            else
+                far_range<KIND, TPL, true>(src, s0, s1, x, y, z, nx, ny, nz, incl, px, py, pz);
+#pragma unroll
+            for (int t = 0; t < TPL; t++) {
+                ax[t] += (double)px[t];
+                ay[t] += (double)py[t];
+                az[t] += (double)pz[t];
+            }
+        }
+    }
+    store_far<TPL>(G, lane, ax, ay, az, out, coef, accumulate);
+}
+
+template <int KIND, int TPL>
+static void far_dispatch_prec(int prec, const DevGeom &g, const void *src, const float *trg32, int ngroups,
+                              const int *go, const FarGroup *gr, const FarInterval *iv, const int *active,
+                              double *out, double coef, int acc, hipStream_t s) {
+    if (prec == 0)
+        k_far_f64<KIND, TPL><<<dim3(ngroups), dim3(64), 0, s>>>((const double *)src, g.coords, g.norms, go, gr, iv,
+                                                               active, out, coef, acc);
+    else
+        k_far_f32<KIND, TPL><<<dim3(ngroups), dim3(64), 0, s>>>((const float *)src, trg32, go, gr, iv, active, out,
+                                                               coef, acc);
+}
+
+template <int KIND>
+static void far_dispatch_tpl(int tpl, int prec, const DevGeom &g, const void *src, const float *trg32, int ngroups,
+                             const int *go, const FarGroup *gr, const FarInterval *iv, const int *active,
+                             double *out, double coef, int acc, hipStream_t s) {
+    switch (tpl) {
+    case 1: far_dispatch_prec<KIND, 1>(prec, g, src, trg32, ngroups, go, gr, iv, active, out, coef, acc, s); break;
+    case 2: far_dispatch_prec<KIND, 2>(prec, g, src, trg32, ngroups, go, gr, iv, active, out, coef, acc, s); break;
+    case 4: far_dispatch_prec<KIND, 4>(prec, g, src, trg32, ngroups, go, gr, iv, active, out, coef, acc, s); break;
+    default: throw Error(-1, "unsupported targets-per-lane");
+    }
+}
+
+void launch_far(int kind, int prec, int tpl, const DevGeom &g, const void *src, const float *trg32, int ngroups,
+                const int *go, const FarGroup *gr, const FarInterval *iv, const int *active, double *out,
+                double coef, int acc, hipStream_t s) {
+    if (ngroups <= 0) return;
+    if (kind == 0)
+        far_dispatch_tpl<0>(tpl, prec, g, src, trg32, ngroups, go, gr, iv, active, out, coef, acc, s);
+    else
+        far_dispatch_tpl<1>(tpl, prec, g, src, trg32, ngroups, go, gr, iv, active, out, coef, acc, s);
+}
+
+// ---------------------------------------------------------------------------
+// SH analysis: coef[S] = A_p * (q_S^-1 in_S), A_p column-major (6Q x 3n).
+
+__global__ __launch_bounds__(256) void k_analysis(DevGeom g, const double *__restrict__ in,
+                                                  double *__restrict__ coef, const int *__restrict__ active) {
+    extern __shared__ double vb[];
+    const int S = blockIdx.x;
+    if (active && !active[g.sph_susp[S]]) return;
+    const int p = g.order[S];
+    const int n = (p + 1) * (2 * p + 2);
+    const int rows = 3 * (p + 1) * (p + 2);
+    const int cols = 3 * n;
+    const int64_t off = g.pt_off[S];
+    const double *q = g.quat + 4 * S;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double v[3] = {in[3 * (off + i)], in[3 * (off + i) + 1], in[3 * (off + i) + 2]};
+        double o[3];
+        qrot(q, v, o, true);
+        vb[3 * i] = o[0];
+        vb[3 * i + 1] = o[1];
+        vb[3 * i + 2] = o[2];
+    }
+    __syncthreads();
+    const double *__restrict__ A = g.anal[p];
+    double *cS = coef + g.coef_off[S];
+    for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+        int c = 0;
+        for (; c + 3 < cols; c += 4) {
+            a0 = __builtin_fma(A[(int64_t)c * rows + r], vb[c], a0);
+            a1 = __builtin_fma(A[(int64_t)(c + 1) * rows + r], vb[c + 1], a1);
+            a2 = __builtin_fma(A[(int64_t)(c + 2) * rows + r], vb[c + 2], a2);
+            a3 = __builtin_fma(A[(int64_t)(c + 3) * rows + r], vb[c + 3], a3);
+        }
+        for (; c < cols; c++) a0 = __builtin_fma(A[(int64_t)c * rows + r], vb[c], a0);
+        cS[r] = (a0 + a1) + (a2 + a3);
+    }
+}
+
+void launch_analysis(const DevGeom &g, const double *in, double *coef, const int *active, hipStream_t s,
+                     int max_order) {
+    if (!g.nsph) return;
+    const size_t lds = sizeof(double) * 3 * (max_order + 1) * (2 * max_order + 2);
+    k_analysis<<<dim3(g.nsph), dim3(256), lds, s>>>(g, in, coef, active);
+}
+
+// ---------------------------------------------------------------------------
+// Expansion records.  Harmonic functions H = sum_{n,m>=0} eps_m Re(h_nm D_n^m).
+// Cartesian derivatives raise the degree by one:
+//   z: h'_{n+1,k} = h_{n,k}
+//   x: h'_{n+1,0} = -h_{n,1},    h'_{n+1,k} = (h_{n,k-1} - h_{n,k+1})/2
+//   y: h'_{n+1,0} = -i h_{n,1},  h'_{n+1,k} = -i (h_{n,k-1} + h_{n,k+1})/2
+// (m=0 coefficients are kept real).  Exterior single-layer flow of a unit-sphere
+// density with vector-SH coefficients (F_r, F_s, F_t), per degree n>=1:
+//   a = n(F_r+(n+3)F_s)/(2(2n+1)(2n+3)), b = n(F_r+(n+1)F_s)/(2n+1), alpha = F_t/(2n+1)
+//   H1 = a, H2 = A_n b, H3 = B_n b, H4 = alpha, P = b,
+//   A_n = (2-n)/(2n(2n-1)), B_n = (n+1)/(n(2n-1))
+//   u = grad H1 + r^2 grad H2 + x H3 + x cross grad H4,  pressure P.
+// Self synthesis of the projected density at r=1:
+//   P sigma = Qr x + grad Qs + x cross grad Qt,  Qr = F_r + (n+1) F_s, Qs = F_s, Qt = F_t.
+// Records store, per (n,m) with n <= p+2, the K arrays' coefficients interleaved
+// as (eps_m Re c, -eps_m Im c) so that eps_m Re(c D) = c0 D.re + c1 D.im.
+
+__device__ __forceinline__ double2 hget(const double2 *h, int P, int n, int k) {
+    if (k < 0 || k > n || n > P || n < 0) return make_double2(0.0, 0.0);
+    return h[dtri(n, k)];
+}
+
+// coefficient (n1, k) of the derivative along dir of the array h (degree <= P)
+__device__ __forceinline__ double2 dcoef(const double2 *h, int P, int n1, int k, int dir) {
+    if (n1 < 1) return make_double2(0.0, 0.0);
+    const int n = n1 - 1;
+    double2 r;
+    if (dir == 2) {
+        r = hget(h, P, n, k);
+    } else if (dir == 0) {
+        if (k == 0) {
+            const double2 a = hget(h, P, n, 1);
+            r = make_double2(-a.x, -a.y);
+        } else {
+            const double2 a = hget(h, P, n, k - 1), b = hget(h, P, n, k + 1);
+            r = make_double2(0.5 * (a.x - b.x), 0.5 * (a.y - b.y));
+        }
+    } else {
+        if (k == 0) {
+            const double2 a = hget(h, P, n, 1); // -i a
+            r = make_double2(a.y, -a.x);
+        } else {
+            const double2 a = hget(h, P, n, k - 1), b = hget(h, P, n, k + 1);
+            const double2 s = make_double2(a.x + b.x, a.y + b.y); // -i s / 2
+            r = make_double2(0.5 * s.y, -0.5 * s.x);
+        }
+    }
+    if (k == 0) r.y = 0.0;
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_expand(DevGeom g, int mode, const double *__restrict__ coef,
+                                                double *__restrict__ rec, const int *__restrict__ active) {
+    extern __shared__ double2 lds[];
+    const int S = blockIdx.x;
+    if (active && !active[g.sph_susp[S]]) return;
+    const int p = g.order[S];
+    const int Q = (p + 1) * (p + 2) / 2, P1 = p + 1, Q1 = (P1 + 1) * (P1 + 2) / 2, P2 = p + 2,
+              Q2 = (P2 + 1) * (P2 + 2) / 2;
+    const int nb = (mode == 0) ? 4 : 8;
+    double2 *base = lds;           // nb arrays of Q
+    double2 *grad = lds + nb * Q;  // (mode 0: H1,H2,H4; mode 1: H1,H2,H3,H4,Qs,Qt) x 3 arrays of Q1
+    const double2 *F = reinterpret_cast<const double2 *>(coef + g.coef_off[S]);
+    // base arrays
+    for (int k = threadIdx.x; k < Q; k += blockDim.x) {
+        int n = 0;
+        while ((n + 1) * (n + 2) / 2 <= k) n++;
+        const double2 fr = F[k], fs = F[Q + k], ft = F[2 * Q + k];
+        double2 h1 = make_double2(0, 0), h2 = h1, h3 = h1, h4 = h1, hp = h1;
+        if (n >= 1) {
+            const double nn = n;
+            const double An = (2.0 - nn) / (2.0 * nn * (2.0 * nn - 1.0));
+            const double Bn = (nn + 1.0) / (nn * (2.0 * nn - 1.0));
+            const double ca = nn / (2.0 * (2.0 * nn + 1.0) * (2.0 * nn + 3.0));
+            const double cb = nn / (2.0 * nn + 1.0);
+            const double2 a = make_double2(ca * (fr.x + (nn + 3.0) * fs.x), ca * (fr.y + (nn + 3.0) * fs.y));
+            const double2 b = make_double2(cb * (fr.x + (nn + 1.0) * fs.x), cb * (fr.y + (nn + 1.0) * fs.y));
+            h1 = a;
+            h2 = make_double2(An * b.x, An * b.y);
+            h3 = make_double2(Bn * b.x, Bn * b.y);
+            h4 = make_double2(ft.x / (2.0 * nn + 1.0), ft.y / (2.0 * nn + 1.0));
+            hp = b;
+        }
+        base[k] = h1;
+        base[Q + k] = h2;
+        base[2 * Q + k] = h3;
+        base[3 * Q + k] = h4;
+        if (mode == 1) {
+            base[4 * Q + k] = hp;
+            base[5 * Q + k] = make_double2(fr.x + (n + 1.0) * fs.x, fr.y + (n + 1.0) * fs.y);
+            base[6 * Q + k] = fs;
+            base[7 * Q + k] = ft;
+        }
+    }
+    __syncthreads();
+    // gradient arrays
+    const int ng = (mode == 0) ? 3 : 6;
+    const int gsrc0[3] = {0, 1, 3};
+    const int gsrc1[6] = {0, 1, 2, 3, 6, 7};
+    for (int idx = threadIdx.x; idx < ng * 3 * Q1; idx += blockDim.x) {
+        const int a = idx / (3 * Q1), rem = idx % (3 * Q1), dir = rem / Q1, k1 = rem % Q1;
+        int n1 = 0;
+        while ((n1 + 1) * (n1 + 2) / 2 <= k1) n1++;
+        const int kk = k1 - n1 * (n1 + 1) / 2;
+        const int f = (mode == 0) ? gsrc0[a] : gsrc1[a];
+        grad[(a * 3 + dir) * Q1 + k1] = dcoef(base + f * Q, p, n1, kk, dir);
+    }
+    __syncthreads();
+    // record
+    const int K = (mode == 0) ? kRecSL : kRecTrac;
+    double *R = rec + g.rec_off[S];
+    const int hd0[6] = {0, 0, 0, 1, 1, 2}, hd1[6] = {0, 1, 2, 1, 2, 2};
+    for (int idx = threadIdx.x; idx < K * Q2; idx += blockDim.x) {
+        const int k2 = idx / K, a = idx % K;
+        int n2 = 0;
+        while ((n2 + 1) * (n2 + 2) / 2 <= k2) n2++;
+        const int m = k2 - n2 * (n2 + 1) / 2;
+        double2 c = make_double2(0, 0);
+        if (mode == 0) {
+            // 0-2 grad H1, 3-5 grad H2, 6 H3, 7-9 grad H4
+            if (a < 6) {
+                const int gi = a / 3, d = a % 3;
+                c = hget(grad + (gi * 3 + d) * Q1, P1, n2, m);
+            } else if (a == 6) {
+                c = hget(base + 2 * Q, p, n2, m);
+            } else {
+                c = hget(grad + (2 * 3 + (a - 7)) * Q1, P1, n2, m);
+            }
+        } else {
+            // 0-5 Hess H1, 6-8 grad H2, 9-14 Hess H2, 15 H3, 16-18 grad H3, 19-24 Hess H4, 25 P,
+            // 26 Qr, 27-29 grad Qs, 30-32 grad Qt
+            if (a < 6) {
+                c = dcoef(grad + (0 * 3 + hd0[a]) * Q1, P1, n2, m, hd1[a]);
+            } else if (a < 9) {
+                c = hget(grad + (1 * 3 + (a - 6)) * Q1, P1, n2, m);
+            } else if (a < 15) {
+                c = dcoef(grad + (1 * 3 + hd0[a - 9]) * Q1, P1, n2, m, hd1[a - 9]);
+            } else if (a == 15) {
+                c = hget(base + 2 * Q, p, n2, m);
+            } else if (a < 19) {
+                c = hget(grad + (2 * 3 + (a - 16)) * Q1, P1, n2, m);
+            } else if (a < 25) {
+                c = dcoef(grad + (3 * 3 + hd0[a - 19]) * Q1, P1, n2, m, hd1[a - 19]);
+            } else if (a == 25) {
+                c = hget(base + 4 * Q, p, n2, m);
+            } else if (a == 26) {
+                c = hget(base + 5 * Q, p, n2, m);
+            } else if (a < 30) {
+                c = hget(grad + (4 * 3 + (a - 27)) * Q1, P1, n2, m);
+            } else {
+                c = hget(grad + (5 * 3 + (a - 30)) * Q1, P1, n2, m);
+            }
+        }
+        const double eps = m ? 2.0 : 1.0;
+        R[2 * (k2 * K + a)] = eps * c.x;
+        R[2 * (k2 * K + a) + 1] = -eps * c.y;
+    }
+}
+
+void launch_expand(const DevGeom &g, int mode, const double *coef, double *rec, const int *active, hipStream_t s,
+                   int max_order) {
+    if (!g.nsph) return;
+    const int Q = tri_size(max_order), Q1 = tri_size(max_order + 1);
+    const size_t lds = sizeof(double) * 2 * ((mode == 0 ? 4 : 8) * Q + (mode == 0 ? 9 : 18) * Q1);
+    static size_t lds_set = 0;
+    if (lds > 65536 && lds > lds_set) {
+        if (hipFuncSetAttribute((const void *)k_expand, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+            hipSuccess)
+            throw Error(-2, "cannot raise the LDS limit of the expansion kernel to " + std::to_string(lds));
+        lds_set = lds;
+    }
+    k_expand<<<dim3(g.nsph), dim3(256), lds, s>>>(g, mode, coef, rec, active);
+}
+
+// ---------------------------------------------------------------------------
+// Expansion evaluation: acc[a] = sum_{n<=P, m<=n} (c0 D.re + c1 D.im) for arrays
+// a in [KBEG, KBEG+KCNT) of a record with KTOT arrays, D_n^m by recurrence.
+
+template <int KTOT, int KBEG, int KCNT>
+__device__ __forceinline__ void eval_record(const double *__restrict__ rec, int P, double x, double y, double z,
+                                            double *acc) {
+#pragma unroll
+    for (int a = 0; a < KCNT; a++) acc[a] = 0.0;
+    const double r2 = x * x + y * y + z * z;
+    const double ir2 = 1.0 / r2;
+    double dmr = rsqrt_f64(r2), dmi = 0.0; // D_m^m
+    for (int m = 0; m <= P; m++) {
+        if (m > 0) {
+            const double f = -(2.0 * m - 1.0) * ir2;
+            const double nr = f * (dmr * x - dmi * y), ni = f * (dmr * y + dmi * x);
+            dmr = nr;
+            dmi = ni;
+        }
+        double pr = 0.0, pi = 0.0, cr = dmr, ci = dmi;
+        const double mm = (double)m * m;
+        for (int n = m; n <= P; n++) {
+            const double *c = rec + 2 * (KTOT * dtri(n, m) + KBEG);
+#pragma unroll
+            for (int a = 0; a < KCNT; a++) acc[a] = __builtin_fma(c[2 * a], cr, __builtin_fma(c[2 * a + 1], ci, acc[a]));
+            const double f1 = -(2.0 * n + 1.0) * z * ir2, f2 = -((double)n * n - mm) * ir2;
+            const double nr = __builtin_fma(f1, cr, f2 * pr), ni = __builtin_fma(f1, ci, f2 * pi);
+            pr = cr;
+            pi = ci;
+            cr = nr;
+            ci = ni;
+        }
+    }
+}
+
+__device__ __forceinline__ void cross3(const double a[3], const double b[3], double o[3]) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// symmetric 3x3 from (xx,xy,xz,yy,yz,zz) times v
+__device__ __forceinline__ void symv(const double *h, const double v[3], double o[3]) {
+    o[0] = h[0] * v[0] + h[1] * v[1] + h[2] * v[2];
+    o[1] = h[1] * v[0] + h[3] * v[1] + h[4] * v[2];
+    o[2] = h[2] * v[0] + h[4] * v[1] + h[5] * v[2];
+}
+
+// traction of the exterior flow from the 26 traction arrays (layout of k_expand)
+__device__ __forceinline__ void traction_from(const double *A, const double x[3], const double v[3], double t[3]) {
+    const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+    const double xv = x[0] * v[0] + x[1] * v[1] + x[2] * v[2];
+    const double *g2 = A + 6, *g3 = A + 16;
+    const double h3 = A[15], pr = A[25];
+    const double vg2 = v[0] * g2[0] + v[1] * g2[1] + v[2] * g2[2];
+    const double vg3 = v[0] * g3[0] + v[1] * g3[1] + v[2] * g3[2];
+    double H1v[3], H2v[3], H4v[3], vx[3], H4vx[3], xH4v[3];
+    symv(A + 0, v, H1v);
+    symv(A + 9, v, H2v);
+    symv(A + 19, v, H4v);
+    cross3(v, x, vx);
+    symv(A + 19, vx, H4vx);
+    cross3(x, H4v, xH4v);
+    for (int c = 0; c < 3; c++)
+        t[c] = -pr * v[c] + 2.0 * H1v[c] + 2.0 * xv * g2[c] + 2.0 * x[c] * vg2 + 2.0 * r2 * H2v[c] +
+               2.0 * h3 * v[c] + x[c] * vg3 + xv * g3[c] + xH4v[c] + H4vx[c];
+}
+
+// MODE 0: single layer, 1: traction, 2: synthesis of the projected density (self only)
+template <int MODE>
+__global__ __launch_bounds__(128) void k_nearself(DevGeom g, const double *__restrict__ rec,
+                                                  const double *__restrict__ in, double *__restrict__ out,
+                                                  double cSL, double cTrac, double cId,
+                                                  const int *__restrict__ active) {
+    const int T = blockIdx.x;
+    if (active && !active[g.sph_susp[T]]) return;
+    const int64_t off = g.pt_off[T];
+    const int nT = (int)(g.pt_off[T + 1] - off);
+    const int nnear = (MODE == 2) ? 0 : (int)(g.near_off[T + 1] - g.near_off[T]);
+    for (int j = threadIdx.x; j < nT; j += blockDim.x) {
+        const int64_t gp = off + j;
+        const double X[3] = {g.coords[3 * gp], g.coords[3 * gp + 1], g.coords[3 * gp + 2]};
+        const double Nl[3] = {g.norms[3 * gp], g.norms[3 * gp + 1], g.norms[3 * gp + 2]};
+        double o[3] = {0.0, 0.0, 0.0};
+        for (int si = 0; si <= nnear; si++) {
+            const int S = (si == 0) ? T : g.near_src[g.near_off[T] + si - 1];
+            const double *q = g.quat + 4 * S;
+            const double R = g.rad[S];
+            const double iR = 1.0 / R;
+            const double d[3] = {(X[0] - g.pos[3 * S]) * iR, (X[1] - g.pos[3 * S + 1]) * iR,
+                                 (X[2] - g.pos[3 * S + 2]) * iR};
+            double xi[3], vb[3], res[3], lab[3];
+            qrot(q, d, xi, true);
+            const int P = g.order[S] + 2;
+            const double *recS = rec + g.rec_off[S];
+            if (MODE == 0) {
+                double A[kRecSL];
+                eval_record<kRecSL, 0, kRecSL>(recS, P, xi[0], xi[1], xi[2], A);
+                const double r2 = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+                double xg4[3];
+                cross3(xi, A + 7, xg4);
+                for (int c = 0; c < 3; c++) res[c] = A[c] + r2 * A[3 + c] + xi[c] * A[6] + xg4[c];
+                qrot(q, res, lab, false);
+                for (int c = 0; c < 3; c++) o[c] += cSL * R * lab[c];
+            } else if (MODE == 1) {
+                qrot(q, Nl, vb, true);
+                if (si == 0) {
+                    double A[kRecTrac];
+                    eval_record<kRecTrac, 0, kRecTrac>(recS, P, xi[0], xi[1], xi[2], A);
+                    traction_from(A, xi, vb, res);
+                    double xq[3];
+                    cross3(xi, A + 30, xq);
+                    for (int c = 0; c < 3; c++) res[c] += 0.5 * (A[26] * xi[c] + A[27 + c] + xq[c]);
+                } else {
+                    double A[kRecTracNear];
+                    eval_record<kRecTrac, 0, kRecTracNear>(recS, P, xi[0], xi[1], xi[2], A);
+                    traction_from(A, xi, vb, res);
+                }
+                qrot(q, res, lab, false);
+                for (int c = 0; c < 3; c++) o[c] += cTrac * lab[c];
+            } else {
+                double A[7];
+                eval_record<kRecTrac, 26, 7>(recS, P, xi[0], xi[1], xi[2], A);
+                double xq[3];
+                cross3(xi, A + 4, xq);
+                for (int c = 0; c < 3; c++) res[c] = A[0] * xi[c] + A[1 + c] + xq[c];
+                qrot(q, res, lab, false);
+                for (int c = 0; c < 3; c++) o[c] = lab[c];
+            }
+        }
+        if (MODE == 2) {
+            for (int c = 0; c < 3; c++) out[3 * gp + c] = o[c];
+        } else {
+            for (int c = 0; c < 3; c++) out[3 * gp + c] += o[c] + cId * in[3 * gp + c];
+        }
+    }
+}
+
+void launch_nearself(const DevGeom &g, int mode, const double *rec, const double *in, double *out, double cSL,
+                     double cTrac, double cId, const int *active, hipStream_t s) {
+    if (!g.nsph) return;
+    if (mode == 0)
+        k_nearself<0><<<dim3(g.nsph), dim3(128), 0, s>>>(g, rec, in, out, cSL, cTrac, cId, active);
+    else
+        k_nearself<1><<<dim3(g.nsph), dim3(128), 0, s>>>(g, rec, in, out, cSL, cTrac, cId, active);
+}
+
+void launch_synthesis(const DevGeom &g, const double *rec, double *out, hipStream_t s) {
+    if (!g.nsph) return;
+    k_nearself<2><<<dim3(g.nsph), dim3(128), 0, s>>>(g, rec, nullptr, out, 0.0, 0.0, 0.0, nullptr);
+}
+
+__global__ void k_identity(DevGeom g, const double *__restrict__ in, double *__restrict__ out, double cId,
+                           const int *__restrict__ active) {
+    const int S = blockIdx.x;
+    if (active && !active[g.sph_susp[S]]) return;
+    for (int64_t i = 3 * g.pt_off[S] + threadIdx.x; i < 3 * g.pt_off[S + 1]; i += blockDim.x) out[i] += cId * in[i];
+}
+
+void launch_identity(const DevGeom &g, const double *in, double *out, double cId, const int *active,
+                     hipStream_t s) {
+    if (!g.nsph) return;
+    k_identity<<<dim3(g.nsph), dim3(256), 0, s>>>(g, in, out, cId, active);
+}
+
+// ---------------------------------------------------------------------------
+// rigid-body projector L (SphereSTKSHOperator.cpp:487-521):
+//   A = sum w sigma / (4 pi R^2),  B = 3/(8 pi R^4) sum w R n x sigma,
+//   out += cLOP (A + B x R n)
+
+__global__ __launch_bounds__(256) void k_lop(DevGeom g, const double *__restrict__ in, double *__restrict__ out,
+                                             double cLOP, const int *__restrict__ active) {
+    __shared__ double sh[8];
+    const int S = blockIdx.x;
+    if (active && !active[g.sph_susp[S]]) return;
+    const int64_t b = g.pt_off[S], e = g.pt_off[S + 1];
+    const double R = g.rad[S];
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+        const double w = g.w[i];
+        const double v[3] = {in[3 * i], in[3 * i + 1], in[3 * i + 2]};
+        const double nv[3] = {g.norms[3 * i], g.norms[3 * i + 1], g.norms[3 * i + 2]};
+        a[0] += w * v[0];
+        a[1] += w * v[1];
+        a[2] += w * v[2];
+        a[3] += w * R * (nv[1] * v[2] - nv[2] * v[1]);
+        a[4] += w * R * (nv[2] * v[0] - nv[0] * v[2]);
+        a[5] += w * R * (nv[0] * v[1] - nv[1] * v[0]);
+    }
+    double tot[6];
+    for (int c = 0; c < 6; c++) tot[c] = block_sum(a[c], sh);
+    const double ca = 1.0 / (4.0 * kPi * R * R), cb = 3.0 / (8.0 * kPi * R * R * R * R);
+    const double A[3] = {tot[0] * ca, tot[1] * ca, tot[2] * ca};
+    const double B[3] = {tot[3] * cb, tot[4] * cb, tot[5] * cb};
+    for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+        const double rn[3] = {R * g.norms[3 * i], R * g.norms[3 * i + 1], R * g.norms[3 * i + 2]};
+        out[3 * i] += cLOP * (A[0] + B[1] * rn[2] - B[2] * rn[1]);
+        out[3 * i + 1] += cLOP * (A[1] + B[2] * rn[0] - B[0] * rn[2]);
+        out[3 * i + 2] += cLOP * (A[2] + B[0] * rn[1] - B[1] * rn[0]);
+    }
+}
+
+void launch_lop(const DevGeom &g, const double *in, double *out, double cLOP, const int *active, hipStream_t s) {
+    if (!g.nsph) return;
+    k_lop<<<dim3(g.nsph), dim3(256), 0, s>>>(g, in, out, cLOP, active);
+}
+
+// ---------------------------------------------------------------------------
+// segmented BLAS-1 (deterministic: per-chunk partials reduced in chunk order)
+
+__global__ __launch_bounds__(256) void k_seg_dot(const double *__restrict__ a, const double *__restrict__ b,
+                                                 const Chunk *__restrict__ ch, double *__restrict__ partial,
+                                                 const int *__restrict__ active) {
+    __shared__ double sh[4];
+    const Chunk c = ch[blockIdx.x];
+    double s = 0.0;
+    if (!active || active[c.seg])
+        for (int64_t i = c.begin + threadIdx.x; i < c.end; i += blockDim.x) s = __builtin_fma(a[i], b[i], s);
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void k_seg_reduce(const double *__restrict__ partial, const int *__restrict__ off, int nseg,
+                             double *__restrict__ out, int op, const int *__restrict__ active) {
+    const int sgi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sgi >= nseg) return;
+    if (active && !active[sgi]) return;
+    double s = 0.0;
+    for (int k = off[sgi]; k < off[sgi + 1]; k++) s += partial[k];
+    if (op == 1) s = (s > 0.0) ? 1.0 / sqrt(s) : 0.0;
+    if (op == 2) s = -s;
+    out[sgi] = s;
+}
+
+__global__ __launch_bounds__(256) void k_seg_axpy(double *__restrict__ y, const double *__restrict__ x,
+                                                  const double *__restrict__ alpha, double sign,
+                                                  const Chunk *__restrict__ ch, const int *__restrict__ active) {
+    const Chunk c = ch[blockIdx.x];
+    if (active && !active[c.seg]) return;
+    const double a = sign * alpha[c.seg];
+    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += blockDim.x) y[i] = __builtin_fma(a, x[i], y[i]);
+}
+
+__global__ __launch_bounds__(256) void k_seg_scale(double *__restrict__ y, const double *__restrict__ x,
+                                                   const double *__restrict__ alpha, const Chunk *__restrict__ ch,
+                                                   const int *__restrict__ active) {
+    const Chunk c = ch[blockIdx.x];
+    if (active && !active[c.seg]) return;
+    const double a = alpha[c.seg];
+    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += blockDim.x) y[i] = a * x[i];
+}
+
+__global__ __launch_bounds__(256) void k_seg_multi_axpy(double *__restrict__ x, const double *__restrict__ V,
+                                                        int64_t ldv, const double *__restrict__ coef, int k,
+                                                        int nseg, const Chunk *__restrict__ ch,
+                                                        const int *__restrict__ active) {
+    const Chunk c = ch[blockIdx.x];
+    if (active && !active[c.seg]) return;
+    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += blockDim.x) {
+        double s = x[i];
+        for (int j = 0; j < k; j++) s = __builtin_fma(coef[(int64_t)j * nseg + c.seg], V[(int64_t)j * ldv + i], s);
+        x[i] = s;
+    }
+}
+
+__global__ void k_axpby(int64_t n, double a, const double *__restrict__ x, double b, double *__restrict__ y) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    y[i] = (b == 0.0) ? a * x[i] : __builtin_fma(a, x[i], b * y[i]);
+}
+
+__global__ void k_fill(int64_t n, double v, double *__restrict__ y) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) y[i] = v;
+}
+
+void launch_seg_dot(const double *a, const double *b, const Chunk *chunks, int nchunks, double *partial,
+                    const int *active, hipStream_t s) {
+    if (nchunks > 0) k_seg_dot<<<dim3(nchunks), dim3(256), 0, s>>>(a, b, chunks, partial, active);
+}
+void launch_seg_reduce(const double *partial, const int *off, int nseg, double *out, int op, const int *active,
+                       hipStream_t s) {
+    if (nseg > 0) k_seg_reduce<<<dim3((nseg + 63) / 64), dim3(64), 0, s>>>(partial, off, nseg, out, op, active);
+}
+void launch_seg_axpy(double *y, const double *x, const double *alpha, double sign, const Chunk *chunks, int nchunks,
+                     const int *active, hipStream_t s) {
+    if (nchunks > 0) k_seg_axpy<<<dim3(nchunks), dim3(256), 0, s>>>(y, x, alpha, sign, chunks, active);
+}
+void launch_seg_scale(double *y, const double *x, const double *alpha, const Chunk *chunks, int nchunks,
+                      const int *active, hipStream_t s) {
+    if (nchunks > 0) k_seg_scale<<<dim3(nchunks), dim3(256), 0, s>>>(y, x, alpha, chunks, active);
+}
+void launch_seg_multi_axpy(double *x, const double *V, int64_t ldv, const double *coef, int k, int nseg,
+                           const Chunk *chunks, int nchunks, const int *active, hipStream_t s) {
+    if (nchunks > 0 && k > 0)
+        k_seg_multi_axpy<<<dim3(nchunks), dim3(256), 0, s>>>(x, V, ldv, coef, k, nseg, chunks, active);
+}
+void launch_axpby(int64_t n, double a, const double *x, double b, double *y, hipStream_t s) {
+    if (n > 0) k_axpby<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(n, a, x, b, y);
+}
+void launch_fill(int64_t n, double v, double *y, hipStream_t s) {
+    if (n > 0) k_fill<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(n, v, y);
+}
+
+// ---------------------------------------------------------------------------
+// mobility helpers (SphereSTKMobMat::solveMob, BIE/SphereSTKMobMat.cpp:174-192, :261-285)
+
+// rho0 = F/(4 pi R^2) + 3/(8 pi R^4) T x (R n) on every point of sphere S
+__global__ void k_rho0(DevGeom g, const double *__restrict__ force, double *__restrict__ rho) {
+    const int S = blockIdx.x;
+    const double R = g.rad[S];
+    const double a = 1.0 / (4.0 * kPi * R * R), c = 3.0 / (8.0 * kPi * R * R * R * R);
+    const double *f = force + 6 * S, *t = force + 6 * S + 3;
+    for (int64_t i = g.pt_off[S] + threadIdx.x; i < g.pt_off[S + 1]; i += blockDim.x) {
+        const double rn[3] = {R * g.norms[3 * i], R * g.norms[3 * i + 1], R * g.norms[3 * i + 2]};
+        rho[3 * i] = a * f[0] + c * (t[1] * rn[2] - t[2] * rn[1]);
+        rho[3 * i + 1] = a * f[1] + c * (t[2] * rn[0] - t[0] * rn[2]);
+        rho[3 * i + 2] = a * f[2] + c * (t[0] * rn[1] - t[1] * rn[0]);
+    }
+}
+
+// V = sum w u/(4 pi R^2 mu), Omega = 3 sum w (R n x u)/(8 pi R^4 mu)
+__global__ __launch_bounds__(256) void k_rigid_avg(DevGeom g, const double *__restrict__ u, double inv_mu,
+                                                   double *__restrict__ vel) {
+    __shared__ double sh[8];
+    const int S = blockIdx.x;
+    const double R = g.rad[S];
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t i = g.pt_off[S] + threadIdx.x; i < g.pt_off[S + 1]; i += blockDim.x) {
+        const double w = g.w[i];
+        const double v[3] = {u[3 * i], u[3 * i + 1], u[3 * i + 2]};
+        const double nv[3] = {g.norms[3 * i], g.norms[3 * i + 1], g.norms[3 * i + 2]};
+        a[0] += w * v[0];
+        a[1] += w * v[1];
+        a[2] += w * v[2];
+        a[3] += R * w * (nv[1] * v[2] - nv[2] * v[1]);
+        a[4] += R * w * (nv[2] * v[0] - nv[0] * v[2]);
+        a[5] += R * w * (nv[0] * v[1] - nv[1] * v[0]);
+    }
+    double tot[6];
+    for (int c = 0; c < 6; c++) tot[c] = block_sum(a[c], sh);
+    if (threadIdx.x == 0) {
+        const double cv = inv_mu / (4.0 * kPi * R * R), cw = 3.0 * inv_mu / (8.0 * kPi * R * R * R * R);
+        for (int c = 0; c < 3; c++) {
+            vel[6 * S + c] = tot[c] * cv;
+            vel[6 * S + 3 + c] = tot[3 + c] * cw;
+        }
+    }
+}
+
+void launch_rho0(const DevGeom &g, const double *force, double *rho, hipStream_t s) {
+    if (g.nsph) k_rho0<<<dim3(g.nsph), dim3(128), 0, s>>>(g, force, rho);
+}
+void launch_rigid_avg(const DevGeom &g, const double *u, double inv_mu, double *vel, hipStream_t s) {
+    if (g.nsph) k_rigid_avg<<<dim3(g.nsph), dim3(256), 0, s>>>(g, u, inv_mu, vel);
+}
+
+} // namespace sphstk

@@ -1,0 +1,81 @@
+// Device views and launch wrappers of the HIP kernels in kernels.hip.
+#pragma once
+
+#include "common.hpp"
+
+namespace sphstk {
+
+constexpr int kMaxOrder = 32;
+
+// Read-only device view of one (possibly batched) operator's geometry.
+struct DevGeom {
+    int nsph = 0, nsusp = 0;
+    int64_t npts = 0;
+    const double *pos = nullptr;   // 3N
+    const double *rad = nullptr;   // N
+    const double *quat = nullptr;  // 4N (w,x,y,z), unit
+    const int *order = nullptr;    // N
+    const int *sph_susp = nullptr; // N
+    const int64_t *pt_off = nullptr;   // N+1
+    const int64_t *coef_off = nullptr; // N+1 (6Q doubles per sphere)
+    const int64_t *rec_off = nullptr;  // N+1 (2*kRecTrac*tri(p+2) doubles per sphere)
+    const double *coords = nullptr;    // 3P lab
+    const double *norms = nullptr;     // 3P lab
+    const double *w = nullptr;         // P
+    const int64_t *near_off = nullptr; // N+1
+    const int *near_src = nullptr;
+    const double *anal[kMaxOrder + 1] = {}; // analysis matrix per order (column-major)
+};
+
+// Chunks of the DOF vector used by the segmented BLAS-1 kernels.
+struct Chunk {
+    int64_t begin, end;
+    int32_t seg, pad;
+};
+
+void launch_pack_src_f64(const DevGeom &g, const double *in, double *src, hipStream_t s);
+void launch_pack_src_f32(const DevGeom &g, const double *in, float *src, hipStream_t s);
+void launch_pack_trg_f32(const DevGeom &g, float *trg, hipStream_t s);
+
+// far field: out = (accumulate ? out : 0) + coef * sum over far pairs
+void launch_far(int kind, int prec, int tpl, const DevGeom &g, const void *src, const float *trg32, int ngroups,
+                const int *group_order, const FarGroup *groups, const FarInterval *iv, const int *active,
+                double *out, double coef, int accumulate, hipStream_t s);
+
+void launch_analysis(const DevGeom &g, const double *in, double *coef, const int *active, hipStream_t s,
+                     int max_order);
+// mode 0: single-layer record (kRecSL arrays), 1: traction record (kRecTrac arrays)
+void launch_expand(const DevGeom &g, int mode, const double *coef, double *rec, const int *active, hipStream_t s,
+                   int max_order);
+// out += cSL*S_self/near + cTrac*K_self/near + cId*in   (mode as above)
+void launch_nearself(const DevGeom &g, int mode, const double *rec, const double *in, double *out, double cSL,
+                     double cTrac, double cId, const int *active, hipStream_t s);
+void launch_identity(const DevGeom &g, const double *in, double *out, double cId, const int *active,
+                     hipStream_t s);
+void launch_lop(const DevGeom &g, const double *in, double *out, double cLOP, const int *active, hipStream_t s);
+// out = P in (grid values of the band-limited projection) from a traction record
+void launch_synthesis(const DevGeom &g, const double *rec, double *out, hipStream_t s);
+
+// segmented BLAS-1 over chunks (one segment per suspension)
+void launch_seg_dot(const double *a, const double *b, const Chunk *chunks, int nchunks, double *partial,
+                    const int *active, hipStream_t s);
+// out[seg] = sum of partials of seg; op 0: sum, 1: 1/sqrt(sum) (0 if sum==0), 2: -sum
+void launch_seg_reduce(const double *partial, const int *seg_chunk_off, int nseg, double *out, int op,
+                       const int *active, hipStream_t s);
+// y += sign * alpha[seg] * x
+void launch_seg_axpy(double *y, const double *x, const double *alpha, double sign, const Chunk *chunks, int nchunks,
+                     const int *active, hipStream_t s);
+// y = alpha[seg] * x
+void launch_seg_scale(double *y, const double *x, const double *alpha, const Chunk *chunks, int nchunks,
+                      const int *active, hipStream_t s);
+// x += sum_i coef[i*nseg + seg] * V[i]   (i < k)
+void launch_seg_multi_axpy(double *x, const double *V, int64_t ldv, const double *coef, int k, int nseg,
+                           const Chunk *chunks, int nchunks, const int *active, hipStream_t s);
+// y = a*x + b*y (plain, whole vector)
+void launch_axpby(int64_t n, double a, const double *x, double b, double *y, hipStream_t s);
+void launch_fill(int64_t n, double v, double *y, hipStream_t s);
+
+void launch_rho0(const DevGeom &g, const double *force, double *rho, hipStream_t s);
+void launch_rigid_avg(const DevGeom &g, const double *u, double inv_mu, double *vel, hipStream_t s);
+
+} // namespace sphstk

@@ -1,0 +1,201 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 CPU oracle on
+identical seeded inputs.  Tolerances: relative 1e-11 for single operator
+applies (pure FP64 arithmetic in a different order), 1e-10 for full FP64
+mobility solves (the north-star parity bar), 1e-7 for the mixed-precision
+solve against the FP64 oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+import spheresimulator_amd as S
+from conftest import rel, relmax
+
+pytestmark = pytest.mark.gpu
+
+ID = [1.0, 0.0, 0.0, 0.0]
+
+
+def say(msg):
+    print("[case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def quat_of(axis, ang):
+    axis = np.asarray(axis, float) / np.linalg.norm(axis)
+    return np.concatenate([[np.cos(ang / 2)], np.sin(ang / 2) * axis])
+
+
+def small_geometry(seed=0, n=8, orders=(6, 8, 5, 7)):
+    rng = np.random.default_rng(seed)
+    pos = [np.zeros(3)]
+    rad = [1.0]
+    while len(pos) < n:
+        r = rng.uniform(0.6, 1.3)
+        x = rng.uniform(-5, 5, 3)
+        if all(np.linalg.norm(x - q) > r + rq + 0.15 for q, rq in zip(pos, rad)):
+            pos.append(x)
+            rad.append(r)
+    quat = np.array([quat_of(rng.standard_normal(3), rng.uniform(0, 3)) for _ in range(n)])
+    order = np.array([orders[i % len(orders)] for i in range(n)], dtype=np.int32)
+    return np.array(pos), np.array(rad), quat, order
+
+
+@pytest.fixture(scope="module")
+def geo():
+    return small_geometry()
+
+
+@pytest.mark.parametrize("coeffs", [(0.5, 0.0, 1.0), (0.0, 1.0, 0.0), (-0.5, 0.0, -1.0), (0.3, 0.7, -0.4)])
+def test_p2p_parity(geo, coeffs):
+    say("p2p parity coeffs=%s" % (coeffs,))
+    pos, rad, quat, order = geo
+    ref = O.Operator(pos, rad, quat, order)
+    gpu = S.SphereSTKSHOperator(pos, rad, quat, order)
+    assert gpu.n_dof == ref.dof
+    assert gpu.n_near_pairs == len(ref.near_pairs()[0]) > 0
+    x = np.random.default_rng(1).standard_normal(ref.dof)
+    y0 = np.random.default_rng(2).standard_normal(ref.dof)
+    want = ref.apply_p2p(x, y0.copy(), *coeffs)
+    got = gpu.applyP2POP(x, y0.copy(), *coeffs)
+    assert relmax(got, want) < 1e-11
+
+
+def test_grid_and_near_list_match(geo):
+    say("grid/near list")
+    pos, rad, quat, order = geo
+    ref = O.Operator(pos, rad, quat, order)
+    gpu = S.SphereSTKSHOperator(pos, rad, quat, order)
+    c, nv, w, off = ref.grid()
+    g = gpu.grid()
+    assert np.abs(g["coords"] - c.ravel()).max() < 1e-14
+    assert np.abs(g["norms"] - nv.ravel()).max() < 1e-15
+    assert np.abs(g["weights"] - w).max() < 1e-15
+    assert np.array_equal(g["index"], off[:-1]) and np.array_equal(g["length"], np.diff(off))
+    t, s = gpu.near_pairs()
+    tr, sr = ref.near_pairs()
+    assert np.array_equal(t, tr) and np.array_equal(s, sr)
+
+
+def test_apply_lop_project_parity(geo):
+    say("apply/lop/project")
+    pos, rad, quat, order = geo
+    ref = O.Operator(pos, rad, quat, order, 0.5, 0.0, 1.0, 1.0)
+    gpu = S.SphereSTKSHOperator(pos, rad, quat, order, 0.5, 0.0, 1.0, 1.0)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(ref.dof)
+    y0 = rng.standard_normal(ref.dof)
+    assert relmax(gpu.apply(x, y0.copy(), 1.7, -0.3), ref.apply(x, y0.copy(), 1.7, -0.3)) < 1e-11
+    assert relmax(gpu.applyLOP(x, y0.copy(), 0.9), ref.apply_lop(x, y0.copy(), 0.9)) < 1e-13
+    px = gpu.projectNullSpace(x.copy())
+    assert relmax(px, ref.project(x)) < 1e-12
+    assert relmax(gpu.projectNullSpace(px.copy()), px) < 1e-12
+    # two columns at once
+    X = np.concatenate([x, y0])
+    Y = gpu.apply(X)
+    assert relmax(Y[:ref.dof], ref.apply(x)) < 1e-11 and relmax(Y[ref.dof:], ref.apply(y0)) < 1e-11
+
+
+def test_pos_initial_operator_parity(pos_initial):
+    """The reference's own input file posInitial.txt (100 spheres, 130 near pairs)."""
+    say("posInitial.txt operator parity")
+    pos, rad = pos_initial
+    n = len(rad)
+    quat = np.tile(ID, (n, 1))
+    order = np.full(n, 6, dtype=np.int32)
+    ref = O.Operator(pos, rad, quat, order)
+    gpu = S.SphereSTKSHOperator(pos, rad, quat, order)
+    x = np.random.default_rng(4).standard_normal(ref.dof)
+    for cf in ((0.5, 0.0, 1.0), (0.0, 1.0, 0.0)):
+        assert relmax(gpu.applyP2POP(x, np.zeros(ref.dof), *cf), ref.apply_p2p(x, None, *cf)) < 1e-11
+
+
+def test_batched_operator_parity():
+    say("batched operator parity")
+    geos = [small_geometry(seed=s, n=k) for s, k in ((5, 3), (6, 6), (7, 1), (8, 4))]
+    pos = np.concatenate([g[0] for g in geos])
+    rad = np.concatenate([g[1] for g in geos])
+    quat = np.concatenate([g[2] for g in geos])
+    order = np.concatenate([g[3] for g in geos])
+    counts = [len(g[1]) for g in geos]
+    gpu = S.SphereSTKSHOperator(pos, rad, quat, order, 0.5, 0.0, 1.0, 1.0, counts=counts)
+    x = np.random.default_rng(9).standard_normal(gpu.n_dof)
+    y = gpu.apply(x)
+    o = 0
+    for g in geos:
+        ref = O.Operator(*g, 0.5, 0.0, 1.0, 1.0)
+        assert relmax(y[o:o + ref.dof], ref.apply(x[o:o + ref.dof])) < 1e-11
+        o += ref.dof
+
+
+def test_isolated_sphere_closed_form():
+    say("isolated sphere")
+    for prec in (S.PREC_FP64, S.PREC_MIXED):
+        M = S.SphereSTKMobMat([[0.3, -1.0, 2.0]], [0.7], [quat_of([1, 2, 3], 0.7)], [8], viscosity=2.5,
+                              precision=prec)
+        F = np.array([1.0, -2.0, 0.5, 0.3, 0.9, -0.4])
+        v, info = M.solveMob(F)
+        assert info["shortcut"] == 1
+        assert rel(v[:3], F[:3] / (6 * np.pi * 2.5 * 0.7)) < 1e-13
+        assert rel(v[3:], F[3:] / (8 * np.pi * 2.5 * 0.7 ** 3)) < 1e-13
+
+
+def test_mobility_cfg1_fp64_and_mixed():
+    """64 spheres p=8 (the reference's CPU-runnable case): FP64 GPU matches the
+    oracle to 1e-10; mixed precision to 1e-7."""
+    say("cfg1 mobility FP64 + mixed")
+    c = S.configs.config1()
+    ref = O.MobilityOperator(c["pos"], c["radius"], c["quat"], c["order"])
+    vr, ir = ref.solve(c["force"], tol=1e-8)
+    assert ir["converged"] == 1
+    gpu = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"])
+    v, info = gpu.solveMob(c["force"])
+    assert info["converged"] == 1
+    V, W = v.reshape(-1, 6)[:, :3], v.reshape(-1, 6)[:, 3:]
+    Vr, Wr = vr.reshape(-1, 6)[:, :3], vr.reshape(-1, 6)[:, 3:]
+    assert rel(V, Vr) < 1e-10
+    assert rel(W, Wr) < 1e-10
+    assert abs(info["iters"] - ir["iters"]) <= 1
+    mix = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], precision=S.PREC_MIXED)
+    vm, im = mix.solveMob(c["force"])
+    assert im["converged"] == 1 and im["relres"] <= 1e-8
+    assert rel(vm.reshape(-1, 6)[:, :3], Vr) < 1e-7
+    assert rel(vm.reshape(-1, 6)[:, 3:], Wr) < 1e-7
+
+
+def test_batched_mobility_matches_individual_solves():
+    say("batched mobility")
+    geos = [small_geometry(seed=s, n=k, orders=(o,)) for s, k, o in ((11, 5, 6), (12, 9, 8), (13, 2, 10))]
+    pos = np.concatenate([g[0] for g in geos])
+    rad = np.concatenate([g[1] for g in geos])
+    quat = np.concatenate([g[2] for g in geos])
+    order = np.concatenate([g[3] for g in geos])
+    counts = [len(g[1]) for g in geos]
+    F = np.random.default_rng(3).standard_normal(6 * len(rad))
+    for prec, tol in ((S.PREC_FP64, 1e-10), (S.PREC_MIXED, 1e-7)):
+        gpu = S.SphereSTKMobMat(pos, rad, quat, order, counts=counts, precision=prec)
+        v, info = gpu.solveMob(F)
+        assert info["converged"] == 1
+        o = 0
+        for g in geos:
+            k = len(g[1])
+            vr, _ = O.MobilityOperator(*g).solve(F[6 * o:6 * (o + k)], tol=1e-8)
+            assert rel(v[6 * o:6 * (o + k)], vr) < tol
+            o += k
+
+
+def test_warm_start_and_density():
+    say("warm start")
+    pos, rad, quat, order = small_geometry(seed=21, n=5, orders=(6,))
+    F = np.random.default_rng(5).standard_normal(30)
+    gpu = S.SphereSTKMobMat(pos, rad, quat, order)
+    v1, i1 = gpu.solveMob(F)
+    v2, i2 = gpu.solveMob(F)  # x0 = xLast (MobMat.cpp:222, :245)
+    assert i2["iters"] < i1["iters"]
+    assert rel(v2, v1) < 1e-9
+    rho = gpu.getDensitySolution()
+    ref = O.MobilityOperator(pos, rad, quat, order)
+    vr, _ = ref.solve(F, tol=1e-8)
+    assert rel(v1, vr) < 1e-10
+    assert np.isfinite(rho).all() and rho.size == gpu.op.n_dof
