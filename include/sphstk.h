@@ -49,7 +49,8 @@ extern "C" {
 
 /* creation flags */
 #define SPHSTK_FLAG_NONE 0
-#define SPHSTK_FLAG_TIMING 1 /* record HIP-event timings of the far-field kernels */
+#define SPHSTK_FLAG_TIMING 1      /* record HIP-event timings of the far-field kernels */
+#define SPHSTK_FLAG_FP32_DIRECT 2 /* FP32 inner operator: direct far field (no two-level coarse grids) */
 
 typedef struct sphstk_operator sphstk_operator;
 typedef struct sphstk_mobility sphstk_mobility;
@@ -106,6 +107,13 @@ int sphstk_operator_sizes(const sphstk_operator *op, int64_t *n_spheres, int64_t
 int sphstk_operator_apply(sphstk_operator *op, const double *X, double *Y, int ncol, double alpha, double beta);
 int sphstk_operator_apply_device(sphstk_operator *op, const double *dX, double *dY, int ncol, double alpha,
                                  double beta);
+
+/* The FP32 inner operator of the mixed-precision solve, Y := beta Y + alpha A32 X
+ * (one column).  No counterpart in the reference: A32 has the far field in FP32
+ * and, unless SPHSTK_FLAG_FP32_DIRECT was given at creation, two-level (sphere
+ * pairs farther apart than 4 (R_s + R_t) interact through degree-p/2 coarse
+ * grids); self and near terms stay FP64. */
+int sphstk_operator_apply_inner(sphstk_operator *op, const double *X, double *Y, double alpha, double beta);
 
 /* applyP2POP: out += cId in + cSL S[in] + cTrac K'[in]  (cpp:262-485; hpp:204-206). */
 int sphstk_operator_apply_p2p(sphstk_operator *op, const double *in, double *out, double cId, double cSL,

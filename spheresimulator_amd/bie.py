@@ -18,6 +18,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 PREC_FP64 = 0
 PREC_MIXED = 1
 FLAG_TIMING = 1
+FLAG_FP32_DIRECT = 2
 
 
 def _d(a):
@@ -47,7 +48,7 @@ class SphereSTKSHOperator:
     """Y := beta Y + alpha (cId I + cSL S + cTrac K' + cLOP L) X on the GPU."""
 
     def __init__(self, pos, radius, quat, order, cId=0.0, cSL=0.0, cTrac=0.0, cLOP=0.0, counts=None, timing=False,
-                 _handle=None, _owner=None):
+                 fp32_direct=False, _handle=None, _owner=None):
         L = load()
         self._owner = _owner
         if _handle is not None:
@@ -59,8 +60,9 @@ class SphereSTKSHOperator:
             check(L.sphstk_operator_create_batched(len(s.counts), s.counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                                    _d(s.pos), _d(s.radius), _d(s.quat),
                                                    s.order.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                                                   cId, cSL, cTrac, cLOP, FLAG_TIMING if timing else 0,
-                                                   ctypes.byref(h)))
+                                                   cId, cSL, cTrac, cLOP,
+                                                   (FLAG_TIMING if timing else 0) |
+                                                   (FLAG_FP32_DIRECT if fp32_direct else 0), ctypes.byref(h)))
             self._h = h
             self._own = True
         ns, npt, nd, nn = (ctypes.c_int64() for _ in range(4))
@@ -83,6 +85,15 @@ class SphereSTKSHOperator:
         if Y.size != X.size or Y.dtype != np.float64 or not Y.flags.c_contiguous:
             raise ValueError("Y must be a contiguous float64 array of the size of X")
         check(load().sphstk_operator_apply(self._h, _d(X), _d(Y), ncol, alpha, beta))
+        return Y
+
+    def apply_inner(self, X, Y=None, alpha=1.0, beta=0.0):
+        """The FP32 inner operator of the mixed-precision solve (one column)."""
+        X = _f64(X, self.n_dof, "X")
+        Y = np.zeros(self.n_dof) if Y is None else Y
+        if Y.size != self.n_dof or Y.dtype != np.float64 or not Y.flags.c_contiguous:
+            raise ValueError("Y must be a contiguous float64 array of length n_dof")
+        check(load().sphstk_operator_apply_inner(self._h, _d(X), _d(Y), alpha, beta))
         return Y
 
     def applyP2POP(self, inp, out, cId, cSL, cTrac):
@@ -161,7 +172,7 @@ class SphereSTKMobMat:
     """Force/torque (6N) -> velocity/angular velocity (6N) through A = 0.5 I + K' + L."""
 
     def __init__(self, pos, radius, quat, order, viscosity=1.0, counts=None, precision=PREC_FP64, params=None,
-                 timing=False):
+                 timing=False, fp32_direct=False):
         L = load()
         s = _Spheres(pos, radius, quat, order, counts)
         self.n_spheres = s.n
@@ -170,8 +181,9 @@ class SphereSTKMobMat:
         check(L.sphstk_mobility_create_batched(len(s.counts), s.counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                                _d(s.pos), _d(s.radius), _d(s.quat),
                                                s.order.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), viscosity,
-                                               ctypes.byref(self.params), precision, FLAG_TIMING if timing else 0,
-                                               ctypes.byref(h)))
+                                               ctypes.byref(self.params), precision,
+                                               (FLAG_TIMING if timing else 0) |
+                                               (FLAG_FP32_DIRECT if fp32_direct else 0), ctypes.byref(h)))
         self._h = h
         self.op = SphereSTKSHOperator(None, None, None, None, _handle=L.sphstk_mobility_operator(h), _owner=self)
 

@@ -44,12 +44,15 @@ struct FarInterval {
     uint64_t mask[kMaxTPL];
 };
 
-// One far-field target group: up to 64*TPL consecutive targets of one suspension.
+// One far-field work item: up to 64*TPL consecutive targets of one suspension
+// against one slice (`slot`) of their source intervals.
 struct FarGroup {
     int64_t tstart;
+    int64_t iv_begin, iv_end;
     int32_t tcount;
     int32_t susp;
-    int64_t iv_begin, iv_end;
+    int32_t slot;
+    int32_t pad;
 };
 
 // Number of complex arrays in the per-sphere expansion records.
@@ -64,9 +67,25 @@ struct ShTables {
     std::vector<double> w;   // npts unit-sphere weights
     // analysis matrix, column-major: A[c*rows + r], rows = 6Q, cols = 3*npts
     std::vector<double> anal;
+    // synthesis matrix, column-major: rows = 3*npts, cols = 6Q
+    std::vector<double> synth;
 };
 
 const ShTables &sh_tables(int p);
+
+// Order of the coarse grid used by the two-level far field of the FP32 inner operator.
+inline int coarse_order(int p) { return p <= 4 ? p : p / 2 + 1; }
+
+// Per-order resampling matrices between the order-p grid and the coarse grid of
+// order pc = coarse_order(p), both acting on body-frame values (3 per point),
+// column-major:
+//   down (3 nc x 3 n): coarse values of the degree-pc truncation of the projection
+//   up   (3 n x 3 nc): fine values of the degree-pc projection of coarse values
+struct Resample {
+    int p = 0, pc = 0, n = 0, nc = 0;
+    std::vector<double> down, up;
+};
+const Resample &resample_tables(int p);
 void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w);
 
 } // namespace sphstk

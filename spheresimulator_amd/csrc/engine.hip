@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <unordered_map>
@@ -16,6 +17,7 @@ namespace {
 
 constexpr int kMaxSupportedOrder = 20; // LDS of the expansion kernel bounds the order
 constexpr int64_t kChunk = 16384;
+constexpr int64_t kWantItems = 40000; // far-field work items wanted when a problem is small
 
 inline void qrot_host(const double *q, const double *v, double *o) {
     const double w = q[0], ux = q[1], uy = q[2], uz = q[3];
@@ -28,6 +30,9 @@ inline void qrot_host(const double *q, const double *v, double *o) {
 using Mask = std::array<uint64_t, kMaxTPL>;
 
 } // namespace
+
+static std::vector<Chunk> make_chunks(int nsusp, const std::vector<int64_t> &susp_sph_off,
+                                      const std::vector<int64_t> &off, std::vector<int> *seg_off);
 
 // ---------------------------------------------------------------------------
 // Operator set-up: SphereSTKSHOperator::setupDOF / setupFMM / setupNearEval
@@ -192,19 +197,28 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
             geom_.anal[p] = d_anal_[p].p;
         }
 
-    build_far_lists(tpl_);
-
     // segmented-vector chunks (one segment per suspension)
-    std::vector<Chunk> ch;
-    std::vector<int> sco(nsusp_ + 1, 0);
-    for (int b = 0; b < nsusp_; b++) {
-        const int64_t d0 = 3 * pt_off_[susp_sph_off_[b]], d1 = 3 * pt_off_[susp_sph_off_[b + 1]];
-        for (int64_t a = d0; a < d1; a += kChunk) ch.push_back(Chunk{a, std::min(a + kChunk, d1), b, 0});
-        sco[b + 1] = (int)ch.size();
-    }
+    std::vector<int> sco;
+    const std::vector<Chunk> ch = make_chunks(nsusp_, susp_sph_off_, pt_off_, &sco);
     chunks_.upload(ch);
     seg_chunk_off_.upload(sco);
     partial_.alloc(std::max<size_t>(ch.size(), 1));
+
+    int tpl = 1, split = 0;
+    if (const char *v = std::getenv("SPHSTK_FAR_TPL")) tpl = std::atoi(v);
+    if (const char *v = std::getenv("SPHSTK_FAR_SPLIT")) split = std::atoi(v);
+    tpl_ = tpl;
+    two_level_ = (flags_ & 2) == 0;
+    if (const char *v = std::getenv("SPHSTK_TWO_LEVEL")) two_level_ = std::atoi(v) != 0;
+    {
+        std::vector<std::vector<int>> excl(nsph_);
+        for (int t = 0; t < nsph_; t++) {
+            excl[t].push_back(t);
+            for (int64_t k = near_off_[t]; k < near_off_[t + 1]; k++) excl[t].push_back(near_src_[k]);
+            std::sort(excl[t].begin(), excl[t].end());
+        }
+        build_far_lists(lists_fine_, nsusp_, susp_sph_off_, pt_off_, pt_off_, excl, false, tpl, split, kWantItems);
+    }
 
     src64_.alloc(std::max<int64_t>(8 * npts_, 8));
     coef_.alloc(std::max<int64_t>(coef_off_[nsph_], 1));
@@ -225,52 +239,75 @@ Operator::~Operator() {
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-// Far-field work lists.  Targets are cut into groups of 64*tpl consecutive points
-// of one suspension.  Each group gets a list of source intervals: the points of
-// its suspension minus, per lane, its own sphere and the spheres near it (the
-// pairs the reference removes again after the FMM, cpp:332-348, :408-433,
-// :641-690).  Groups are launched in order of decreasing work.
-void Operator::build_far_lists(int tpl) {
+// Far-field work lists.  Targets (a point set with per-sphere offsets `toff`)
+// are cut into groups of 64*tpl consecutive points of one suspension.  Each
+// group gets a list of source intervals over the point set `soff`: in exclude
+// mode all spheres of its suspension minus, per lane, the spheres listed for the
+// lane's target sphere (its own sphere and its near spheres: the pairs the
+// reference removes again after the FMM, cpp:332-348, :408-433, :641-690); in
+// include mode only the listed spheres.  Each group's list is cut into `split`
+// slices of nearly equal length, one work item each, launched in order of
+// decreasing work.
+void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sph_off,
+                     const std::vector<int64_t> &toff, const std::vector<int64_t> &soff,
+                     const std::vector<std::vector<int>> &lists, bool include, int tpl, int split,
+                     int64_t want_items) {
     SPHSTK_REQUIRE(tpl == 1 || tpl == 2 || tpl == 4, "targets per lane must be 1, 2 or 4");
-    tpl_ = tpl;
+    SPHSTK_REQUIRE(split >= 0 && split <= 64, "source split must be in [0, 64] (0 = automatic)");
+    const int nsph = (int)toff.size() - 1;
+    const int64_t ntrg = toff[nsph];
+    std::vector<int32_t> tsph(ntrg);
+    for (int i = 0; i < nsph; i++)
+        for (int64_t k = toff[i]; k < toff[i + 1]; k++) tsph[k] = i;
     const int G = 64 * tpl;
+    int64_t nbase = 0;
+    for (int b = 0; b < nsusp; b++) {
+        const int64_t np = toff[susp_sph_off[b + 1]] - toff[susp_sph_off[b]];
+        nbase += (np + G - 1) / G;
+    }
+    if (split == 0) {
+        const int64_t nb = std::max<int64_t>(nbase, 1);
+        split = (int)std::max<int64_t>(1, std::min<int64_t>(8, (want_items + nb - 1) / nb));
+    }
+    L.tpl = tpl;
+    L.split = split;
+    L.ntrg = ntrg;
     std::vector<FarGroup> groups;
-    std::vector<FarInterval> ivs;
+    std::vector<FarInterval> ivs, giv;
     std::vector<int64_t> work;
     double pairs = 0.0;
     Mask full;
     full.fill(~0ull);
     std::vector<std::pair<int, Mask>> spec;
-    for (int b = 0; b < nsusp_; b++) {
-        const int s0 = (int)susp_sph_off_[b], s1 = (int)susp_sph_off_[b + 1];
-        const int64_t p0 = pt_off_[s0], p1 = pt_off_[s1];
+    for (int b = 0; b < nsusp; b++) {
+        const int s0 = (int)susp_sph_off[b], s1 = (int)susp_sph_off[b + 1];
+        const int64_t p0 = toff[s0], p1 = toff[s1];
         for (int64_t t0 = p0; t0 < p1; t0 += G) {
             const int cnt = (int)std::min<int64_t>(G, p1 - t0);
-            FarGroup g{t0, cnt, b, (int64_t)ivs.size(), 0};
             Mask act{};
             for (int i = 0; i < cnt; i++) act[i / 64] |= 1ull << (i % 64);
             spec.clear();
-            const int Ta = pt_sph_[t0], Tb = pt_sph_[t0 + cnt - 1];
+            giv.clear();
+            const int Ta = tsph[t0], Tb = tsph[t0 + cnt - 1];
             for (int T = Ta; T <= Tb; T++) {
-                const int64_t lo = std::max(pt_off_[T], t0) - t0, hi = std::min(pt_off_[T + 1], t0 + cnt) - t0;
+                const int64_t lo = std::max(toff[T], t0) - t0, hi = std::min(toff[T + 1], t0 + cnt) - t0;
                 Mask lanes{};
                 for (int64_t i = lo; i < hi; i++) lanes[i / 64] |= 1ull << (i % 64);
-                spec.push_back({T, lanes});
-                for (int64_t k = near_off_[T]; k < near_off_[T + 1]; k++) spec.push_back({near_src_[k], lanes});
+                for (int S : lists[T]) spec.push_back({S, lanes});
             }
             std::sort(spec.begin(), spec.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
             int64_t gwork = 0;
             auto emit = [&](int64_t bb, int64_t ee, const Mask &m) {
                 if (bb >= ee) return;
-                if ((int64_t)ivs.size() > g.iv_begin && ivs.back().end == bb &&
-                    std::memcmp(ivs.back().mask, m.data(), sizeof(uint64_t) * kMaxTPL) == 0) {
-                    ivs.back().end = ee;
+                if (!giv.empty() && giv.back().end == bb &&
+                    std::memcmp(giv.back().mask, m.data(), sizeof(uint64_t) * kMaxTPL) == 0) {
+                    giv.back().end = ee;
                 } else {
                     FarInterval iv;
                     iv.begin = bb;
                     iv.end = ee;
                     std::memcpy(iv.mask, m.data(), sizeof(uint64_t) * kMaxTPL);
-                    ivs.push_back(iv);
+                    giv.push_back(iv);
                 }
                 int lanes = 0;
                 for (int w = 0; w < tpl; w++) lanes += __builtin_popcountll(m[w] & act[w]);
@@ -281,46 +318,189 @@ void Operator::build_far_lists(int tpl) {
             size_t k = 0;
             while (k < spec.size()) {
                 const int S = spec[k].first;
-                Mask ex{};
+                Mask sel{};
                 while (k < spec.size() && spec[k].first == S) {
-                    for (int w = 0; w < kMaxTPL; w++) ex[w] |= spec[k].second[w];
+                    for (int w = 0; w < kMaxTPL; w++) sel[w] |= spec[k].second[w];
                     k++;
                 }
                 if (S < s0 || S >= s1) continue;
-                emit(pt_off_[cur], pt_off_[S], full);
                 Mask inc;
                 bool any = false;
                 for (int w = 0; w < kMaxTPL; w++) {
-                    inc[w] = (w < tpl) ? ~ex[w] : ~0ull;
+                    if (w < tpl)
+                        inc[w] = include ? (sel[w] | ~act[w]) : ~sel[w];
+                    else
+                        inc[w] = ~0ull;
                     if (w < tpl && (inc[w] & act[w])) any = true;
                 }
-                if (any) emit(pt_off_[S], pt_off_[S + 1], inc);
+                if (!include) emit(soff[cur], soff[S], full);
+                if (any) emit(soff[S], soff[S + 1], inc);
                 cur = S + 1;
             }
-            emit(pt_off_[cur], pt_off_[s1], full);
-            g.iv_end = (int64_t)ivs.size();
-            groups.push_back(g);
-            work.push_back(gwork);
+            if (!include) emit(soff[cur], soff[s1], full);
+            size_t iv_i = 0;
+            int64_t iv_pos = giv.empty() ? 0 : giv[0].begin;
+            for (int sl = 0; sl < split; sl++) {
+                const int64_t quota = gwork * (sl + 1) / split - gwork * sl / split;
+                FarGroup g{t0, (int64_t)ivs.size(), 0, cnt, b, sl, 0};
+                int64_t left = quota;
+                while (left > 0 && iv_i < giv.size()) {
+                    const int64_t take = std::min(giv[iv_i].end - iv_pos, left);
+                    FarInterval piece = giv[iv_i];
+                    piece.begin = iv_pos;
+                    piece.end = iv_pos + take;
+                    ivs.push_back(piece);
+                    left -= take;
+                    iv_pos += take;
+                    if (iv_pos == giv[iv_i].end) {
+                        iv_i++;
+                        if (iv_i < giv.size()) iv_pos = giv[iv_i].begin;
+                    }
+                }
+                g.iv_end = (int64_t)ivs.size();
+                groups.push_back(g);
+                work.push_back(quota);
+            }
         }
     }
-    // every interval lies inside its group's suspension and every group inside the point range
     for (const FarGroup &g : groups) {
         const int b = g.susp;
-        const int64_t lo = pt_off_[susp_sph_off_[b]], hi = pt_off_[susp_sph_off_[b + 1]];
-        SPHSTK_REQUIRE(g.tstart >= lo && g.tstart + g.tcount <= hi && g.tcount > 0 && g.tcount <= G,
+        const int64_t tlo = toff[susp_sph_off[b]], thi = toff[susp_sph_off[b + 1]];
+        const int64_t slo = soff[susp_sph_off[b]], shi = soff[susp_sph_off[b + 1]];
+        SPHSTK_REQUIRE(g.tstart >= tlo && g.tstart + g.tcount <= thi && g.tcount > 0 && g.tcount <= G &&
+                           g.slot >= 0 && g.slot < split,
                        "internal: far-field group out of range");
         for (int64_t k = g.iv_begin; k < g.iv_end; k++)
-            SPHSTK_REQUIRE(ivs[k].begin >= lo && ivs[k].end <= hi && ivs[k].begin < ivs[k].end,
+            SPHSTK_REQUIRE(ivs[k].begin >= slo && ivs[k].end <= shi && ivs[k].begin < ivs[k].end,
                            "internal: far-field interval out of range");
     }
     std::vector<int> ord(groups.size());
     std::iota(ord.begin(), ord.end(), 0);
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return work[a] > work[b]; });
-    ngroups_ = (int)groups.size();
-    pairs_ = pairs;
-    groups_.upload(groups);
-    iv_.upload(ivs);
-    gorder_.upload(ord);
+    L.ngroups = (int)groups.size();
+    L.pairs = pairs;
+    L.groups.upload(groups);
+    L.iv.upload(ivs);
+    L.order.upload(ord);
+    if (split > 1) L.part.alloc(size_t(split) * 3 * std::max<int64_t>(ntrg, 1));
+}
+
+static std::vector<Chunk> make_chunks(int nsusp, const std::vector<int64_t> &susp_sph_off,
+                                      const std::vector<int64_t> &off, std::vector<int> *seg_off) {
+    std::vector<Chunk> ch;
+    if (seg_off) seg_off->assign(nsusp + 1, 0);
+    for (int b = 0; b < nsusp; b++) {
+        const int64_t d0 = 3 * off[susp_sph_off[b]], d1 = 3 * off[susp_sph_off[b + 1]];
+        for (int64_t a = d0; a < d1; a += kChunk) ch.push_back(Chunk{a, std::min(a + kChunk, d1), b, 0});
+        if (seg_off) (*seg_off)[b + 1] = (int)ch.size();
+    }
+    return ch;
+}
+
+// Two-level far field of the FP32 inner operator: pairs of spheres closer than
+// alpha (R_s + R_t) (and not near) interact point to point on the fine grids;
+// all other far pairs interact between the coarse grids (order coarse_order(p))
+// through the degree-truncated density, and the coarse-grid field is
+// interpolated back to the fine grid.
+void Operator::ensure_two_level() {
+    if (two_level_ready_) return;
+    double alpha = 4.0;
+    if (const char *v = std::getenv("SPHSTK_TWO_LEVEL_ALPHA")) alpha = std::atof(v);
+    SPHSTK_REQUIRE(alpha >= 1.5, "two-level separation factor must be >= 1.5");
+    alpha_ = alpha;
+    // coarse geometry
+    pc_off_.assign(nsph_ + 1, 0);
+    int maxpts = 0;
+    for (int i = 0; i < nsph_; i++) {
+        const Resample &rs = resample_tables(order_[i]);
+        pc_off_[i + 1] = pc_off_[i] + rs.nc;
+        maxpts = std::max(maxpts, std::max(rs.n, rs.nc));
+    }
+    max_rs_points_ = maxpts;
+    const int64_t nc = pc_off_[nsph_];
+    std::vector<double> cc(3 * nc), cn(3 * nc), cw(nc);
+    for (int i = 0; i < nsph_; i++) {
+        const ShTables &t = sh_tables(coarse_order(order_[i]));
+        for (int q = 0; q < t.npts; q++) {
+            const int64_t g = pc_off_[i] + q;
+            double nv[3];
+            qrot_host(&quat_[4 * i], &t.eta[3 * q], nv);
+            for (int c = 0; c < 3; c++) {
+                cn[3 * g + c] = nv[c];
+                cc[3 * g + c] = pos_[3 * i + c] + rad_[i] * nv[c];
+            }
+            cw[g] = rad_[i] * rad_[i] * t.w[q];
+        }
+    }
+    d_pc_off_.upload(pc_off_);
+    d_ccoords_.upload(cc);
+    d_cnorms_.upload(cn);
+    d_cw_.upload(cw);
+    std::vector<bool> used(kMaxOrder + 1, false);
+    for (int p : order_) used[p] = true;
+    for (int p = 0; p <= kMaxOrder; p++)
+        if (used[p]) {
+            const Resample &rs = resample_tables(p);
+            d_down_[p].upload(rs.down);
+            d_up_[p].upload(rs.up);
+            down_tab_.M[p] = d_down_[p].p;
+            down_tab_.nin[p] = rs.n;
+            down_tab_.nout[p] = rs.nc;
+            up_tab_.M[p] = d_up_[p].p;
+            up_tab_.nin[p] = rs.nc;
+            up_tab_.nout[p] = rs.n;
+        }
+    // mid lists: not near, not self, centre distance < alpha (R_s + R_t), same suspension
+    std::vector<std::vector<int>> mid(nsph_), excl(nsph_);
+    for (int b = 0; b < nsusp_; b++) {
+        const int s0 = (int)susp_sph_off_[b], s1 = (int)susp_sph_off_[b + 1];
+        if (s1 <= s0) continue;
+        double rmax = 0, lo[3] = {1e300, 1e300, 1e300};
+        for (int i = s0; i < s1; i++) {
+            rmax = std::max(rmax, rad_[i]);
+            for (int c = 0; c < 3; c++) lo[c] = std::min(lo[c], pos_[3 * i + c]);
+        }
+        const double h = 2.0 * alpha * rmax;
+        std::unordered_map<int64_t, std::vector<int>> cells;
+        auto key = [&](int64_t a, int64_t bb, int64_t c) { return (a * 1000003LL + bb) * 1000033LL + c; };
+        std::vector<std::array<int64_t, 3>> cid(s1 - s0);
+        for (int i = s0; i < s1; i++) {
+            for (int c = 0; c < 3; c++) cid[i - s0][c] = (int64_t)std::floor((pos_[3 * i + c] - lo[c]) / h);
+            cells[key(cid[i - s0][0], cid[i - s0][1], cid[i - s0][2])].push_back(i);
+        }
+        for (int t = s0; t < s1; t++) {
+            const auto &ct = cid[t - s0];
+            for (int dx = -1; dx <= 1; dx++)
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dz = -1; dz <= 1; dz++) {
+                        auto it = cells.find(key(ct[0] + dx, ct[1] + dy, ct[2] + dz));
+                        if (it == cells.end()) continue;
+                        for (int s : it->second) {
+                            if (s == t) continue;
+                            const double ddx = pos_[3 * s] - pos_[3 * t], ddy = pos_[3 * s + 1] - pos_[3 * t + 1],
+                                         ddz = pos_[3 * s + 2] - pos_[3 * t + 2];
+                            const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                            const double rn = 1.5 * (rad_[s] + rad_[t]), rm = alpha * (rad_[s] + rad_[t]);
+                            if (d2 >= rn * rn && d2 < rm * rm) mid[t].push_back(s);
+                        }
+                    }
+            std::sort(mid[t].begin(), mid[t].end());
+            excl[t].push_back(t);
+            for (int64_t k = near_off_[t]; k < near_off_[t + 1]; k++) excl[t].push_back(near_src_[k]);
+            excl[t].insert(excl[t].end(), mid[t].begin(), mid[t].end());
+            std::sort(excl[t].begin(), excl[t].end());
+        }
+    }
+    build_far_lists(lists_mid_, nsusp_, susp_sph_off_, pt_off_, pt_off_, mid, true, tpl_, 0, kWantItems);
+    build_far_lists(lists_coarse_, nsusp_, susp_sph_off_, pc_off_, pc_off_, excl, false, tpl_, 0, kWantItems);
+    std::vector<int> dummy;
+    cchunks_.upload(make_chunks(nsusp_, susp_sph_off_, pc_off_, &dummy));
+    sc_.alloc(std::max<int64_t>(3 * nc, 1));
+    tc_.alloc(std::max<int64_t>(3 * nc, 1));
+    src32c_.alloc(std::max<int64_t>(8 * nc, 8));
+    trg32c_.alloc(std::max<int64_t>(8 * nc, 8));
+    launch_pack_trg_f32(nc, d_ccoords_.p, d_cnorms_.p, trg32c_.p, stream_);
+    two_level_ready_ = true;
 }
 
 void Operator::harvest_events() {
@@ -335,7 +515,23 @@ void Operator::harvest_events() {
     pending_.clear();
 }
 
-void Operator::far(int kind, int prec, double coef, double *out, const int *active) {
+void Operator::far_run(FarLists &L, int kind, int prec, const void *src, const double *tcoords,
+                       const double *tnorms, const float *trg32, const Chunk *chunks, int nchunks, double coef,
+                       double *out, const int *active) {
+    if (L.split > 1) {
+        launch_far(kind, prec, L.tpl, src, tcoords, tnorms, trg32, L.ntrg, L.ngroups, L.order.p, L.groups.p, L.iv.p,
+                   active, L.part.p, coef, 0, stream_);
+        launch_slot_reduce(out, L.part.p, L.split, 3 * L.ntrg, chunks, nchunks, active, stream_);
+    } else {
+        launch_far(kind, prec, L.tpl, src, tcoords, tnorms, trg32, L.ntrg, L.ngroups, L.order.p, L.groups.p, L.iv.p,
+                   active, out, coef, 1, stream_);
+    }
+}
+
+// Far field of `in` into out (accumulating).  FP64: fine grid, direct sum over
+// all far pairs.  FP32: two-level (fine for mid-range pairs, coarse grids for
+// the rest) unless disabled, in which case the FP32 direct sum.
+void Operator::far(int kind, int prec, double coef, const double *in, double *out, const int *active) {
     const bool timing = (flags_ & 1) != 0;
     EvPair ev{};
     if (timing) {
@@ -348,17 +544,50 @@ void Operator::far(int kind, int prec, double coef, double *out, const int *acti
         }
         ev.kind = kind;
         ev.prec = prec;
-        SPHSTK_HIP(hipEventRecord(ev.a, stream_));
     }
-    const void *src = prec == 0 ? (const void *)src64_.p : (const void *)src32_.p;
-    launch_far(kind, prec, tpl_, geom_, src, trg32_.p, ngroups_, gorder_.p, groups_.p, iv_.p, active, out, coef, 1,
-               stream_);
+    if (prec == 0) {
+        launch_pack_src_f64(npts_, d_coords_.p, d_w_.p, in, src64_.p, stream_);
+        if (timing) SPHSTK_HIP(hipEventRecord(ev.a, stream_));
+        far_run(lists_fine_, kind, 0, src64_.p, d_coords_.p, d_norms_.p, nullptr, chunks_.p, (int)chunks_.n, coef,
+                out, active);
+    } else {
+        if (!trg32_.p) {
+            trg32_.alloc(8 * std::max<int64_t>(npts_, 1));
+            src32_.alloc(8 * std::max<int64_t>(npts_, 1));
+            launch_pack_trg_f32(npts_, d_coords_.p, d_norms_.p, trg32_.p, stream_);
+        }
+        launch_pack_src_f32(npts_, d_coords_.p, d_w_.p, in, src32_.p, stream_);
+        if (two_level_) {
+            ensure_two_level();
+            const int64_t nc = pc_off_[nsph_];
+            launch_resample(geom_, down_tab_, d_pt_off_.p, d_pc_off_.p, in, sc_.p, 0, active, stream_,
+                            max_rs_points_);
+            launch_pack_src_f32(nc, d_ccoords_.p, d_cw_.p, sc_.p, src32c_.p, stream_);
+            launch_fill(3 * nc, 0.0, tc_.p, stream_);
+            if (timing) SPHSTK_HIP(hipEventRecord(ev.a, stream_));
+            far_run(lists_coarse_, kind, 1, src32c_.p, nullptr, nullptr, trg32c_.p, cchunks_.p, (int)cchunks_.n,
+                    coef, tc_.p, active);
+            far_run(lists_mid_, kind, 1, src32_.p, nullptr, nullptr, trg32_.p, chunks_.p, (int)chunks_.n, coef, out,
+                    active);
+            if (timing) {
+                SPHSTK_HIP(hipEventRecord(ev.b, stream_));
+                pending_.push_back(ev);
+                timing_done_ = true;
+            }
+            launch_resample(geom_, up_tab_, d_pc_off_.p, d_pt_off_.p, tc_.p, out, 1, active, stream_, max_rs_points_);
+        } else {
+            if (timing) SPHSTK_HIP(hipEventRecord(ev.a, stream_));
+            far_run(lists_fine_, kind, 1, src32_.p, nullptr, nullptr, trg32_.p, chunks_.p, (int)chunks_.n, coef, out,
+                    active);
+        }
+    }
     SPHSTK_HIP(hipGetLastError());
-    if (timing) {
+    if (timing && !timing_done_) {
         SPHSTK_HIP(hipEventRecord(ev.b, stream_));
         pending_.push_back(ev);
-        if (pending_.size() > 64) harvest_events();
     }
+    timing_done_ = false;
+    if (timing && pending_.size() > 64) harvest_events();
 }
 
 // applyP2POP (cpp:262-485), accumulating into out.
@@ -371,24 +600,14 @@ void Operator::p2p(const double *in, double *out, double cId, double cSL, double
     }
     if (nsph_ == 0) return;
     launch_analysis(geom_, in, coef_.p, active, stream_, max_order_);
-    if (prec == 0) {
-        launch_pack_src_f64(geom_, in, src64_.p, stream_);
-    } else {
-        if (!trg32_.p) {
-            trg32_.alloc(8 * npts_);
-            src32_.alloc(8 * npts_);
-            launch_pack_trg_f32(geom_, trg32_.p, stream_);
-        }
-        launch_pack_src_f32(geom_, in, src32_.p, stream_);
-    }
     if (doT) {
-        far(0, prec, -3.0 / (4.0 * kPi) * cTrac, out, active);
+        far(0, prec, -3.0 / (4.0 * kPi) * cTrac, in, out, active);
         launch_expand(geom_, 1, coef_.p, rec_.p, active, stream_, max_order_);
         launch_nearself(geom_, 1, rec_.p, in, out, 0.0, cTrac, cid, active, stream_);
         cid = 0.0;
     }
     if (doSL) {
-        far(1, prec, cSL / (8.0 * kPi), out, active);
+        far(1, prec, cSL / (8.0 * kPi), in, out, active);
         launch_expand(geom_, 0, coef_.p, rec_.p, active, stream_, max_order_);
         launch_nearself(geom_, 0, rec_.p, in, out, cSL, 0.0, cid, active, stream_);
     }
@@ -444,7 +663,12 @@ void Operator::kernel_stats(int kind, int prec, double *ms, int64_t *launches, d
     harvest_events();
     if (ms) *ms = stats_[kind][prec].ms;
     if (launches) *launches = stats_[kind][prec].launches;
-    if (pairs) *pairs = pairs_;
+    if (pairs) {
+        if (prec == 1 && two_level_ && two_level_ready_)
+            *pairs = lists_mid_.pairs + lists_coarse_.pairs;
+        else
+            *pairs = lists_fine_.pairs;
+    }
 }
 
 void Operator::reset_stats() {

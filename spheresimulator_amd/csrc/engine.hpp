@@ -36,6 +36,22 @@ struct DevBuf {
     }
 };
 
+// Far-field work lists over one target point set (see build_far_lists).
+struct FarLists {
+    DevBuf<FarGroup> groups;
+    DevBuf<FarInterval> iv;
+    DevBuf<int> order;
+    DevBuf<double> part; // split partial-sum slots of 3*ntrg doubles
+    int ngroups = 0, tpl = 1, split = 1;
+    double pairs = 0.0; // algorithmic pairs per launch
+    int64_t ntrg = 0;
+};
+
+void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sph_off,
+                     const std::vector<int64_t> &toff, const std::vector<int64_t> &soff,
+                     const std::vector<std::vector<int>> &lists, bool include, int tpl, int split,
+                     int64_t want_items);
+
 struct KernelStats {
     double ms = 0.0;
     int64_t launches = 0;
@@ -88,6 +104,8 @@ class Operator {
     void near_pairs(int32_t *trg, int32_t *src) const;
     void kernel_stats(int kind, int prec, double *ms, int64_t *launches, double *pairs);
     void reset_stats();
+    int far_tpl() const { return tpl_; }
+    int far_split() const { return lists_fine_.split; }
 
     // segmented-vector helpers (one segment per suspension)
     const Chunk *chunks() const { return chunks_.p; }
@@ -98,14 +116,16 @@ class Operator {
     double *scratch_partial() { return partial_.p; }
 
   private:
-    void build_far_lists(int tpl);
-    void far(int kind, int prec, double coef, double *out, const int *active);
+    void far(int kind, int prec, double coef, const double *in, double *out, const int *active);
+    void far_run(FarLists &L, int kind, int prec, const void *src, const double *tcoords, const double *tnorms,
+                 const float *trg32, const Chunk *chunks, int nchunks, double coef, double *out, const int *active);
+    void ensure_two_level();
 
     int nsph_, nsusp_;
     int64_t npts_;
     double c_[4];
     int flags_;
-    int tpl_ = 2;
+    int tpl_ = 1;
     int max_order_ = 0;
     hipStream_t stream_ = nullptr;
     int device_ = 0;
@@ -123,12 +143,19 @@ class Operator {
     DevBuf<double> d_anal_[kMaxOrder + 1];
     DevGeom geom_;
 
-    // far-field work lists
-    DevBuf<FarGroup> groups_;
-    DevBuf<FarInterval> iv_;
-    DevBuf<int> gorder_;
-    int ngroups_ = 0;
-    double pairs_ = 0.0; // algorithmic far pairs per launch
+    // far-field work lists: fine grid (FP64 and direct FP32), and the two-level
+    // FP32 pair (fine mid-range pairs, coarse-grid pairs)
+    FarLists lists_fine_, lists_mid_, lists_coarse_;
+    bool two_level_ = true, two_level_ready_ = false, timing_done_ = false;
+    double alpha_ = 4.0;
+    int max_rs_points_ = 0;
+    std::vector<int64_t> pc_off_;
+    DevBuf<int64_t> d_pc_off_;
+    DevBuf<double> d_ccoords_, d_cnorms_, d_cw_, sc_, tc_;
+    DevBuf<float> src32c_, trg32c_;
+    DevBuf<Chunk> cchunks_;
+    DevBuf<double> d_down_[kMaxOrder + 1], d_up_[kMaxOrder + 1];
+    ResampleTabs down_tab_, up_tab_;
 
     // per-apply scratch
     DevBuf<double> src64_, coef_, rec_, tmp_;

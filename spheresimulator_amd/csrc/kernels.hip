@@ -109,20 +109,16 @@ __global__ void k_pack_trg_f32(int64_t n, const double *__restrict__ coords, con
     o[7] = 0.f;
 }
 
-void launch_pack_src_f64(const DevGeom &g, const double *in, double *src, hipStream_t s) {
-    const int64_t n = g.npts;
-    if (!n) return;
-    k_pack_src_f64<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(n, g.coords, g.w, in, src);
+void launch_pack_src_f64(int64_t n, const double *coords, const double *w, const double *in, double *src,
+                         hipStream_t s) {
+    if (n > 0) k_pack_src_f64<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(n, coords, w, in, src);
 }
-void launch_pack_src_f32(const DevGeom &g, const double *in, float *src, hipStream_t s) {
-    const int64_t n = g.npts;
-    if (!n) return;
-    k_pack_src_f32<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(n, g.coords, g.w, in, src);
+void launch_pack_src_f32(int64_t n, const double *coords, const double *w, const double *in, float *src,
+                         hipStream_t s) {
+    if (n > 0) k_pack_src_f32<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(n, coords, w, in, src);
 }
-void launch_pack_trg_f32(const DevGeom &g, float *trg, hipStream_t s) {
-    const int64_t n = g.npts;
-    if (!n) return;
-    k_pack_trg_f32<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(n, g.coords, g.norms, trg);
+void launch_pack_trg_f32(int64_t n, const double *coords, const double *norms, float *trg, hipStream_t s) {
+    if (n > 0) k_pack_trg_f32<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(n, coords, norms, trg);
 }
 
 // ---------------------------------------------------------------------------
@@ -189,9 +185,13 @@ __device__ __forceinline__ void far_range(const R *__restrict__ src, int64_t b, 
     }
 }
 
+// accumulate != 0: out += coef*acc.  accumulate == 0: out[slot*slot_stride + ...] = coef*acc
+// (one partial-sum slot per source slice, reduced afterwards in slot order).
 template <int TPL>
 __device__ __forceinline__ void store_far(const FarGroup &G, int lane, const double *ax, const double *ay,
-                                          const double *az, double *__restrict__ out, double coef, int accumulate) {
+                                          const double *az, double *__restrict__ out, double coef, int accumulate,
+                                          int64_t slot_stride) {
+    if (!accumulate) out += G.slot * slot_stride;
 #pragma unroll
     for (int t = 0; t < TPL; t++) {
         const int idx = t * 64 + lane;
@@ -215,7 +215,8 @@ __global__ __launch_bounds__(64) void k_far_f64(const double *__restrict__ src, 
                                                 const double *__restrict__ norms, const int *__restrict__ gorder,
                                                 const FarGroup *__restrict__ groups,
                                                 const FarInterval *__restrict__ iv, const int *__restrict__ active,
-                                                double *__restrict__ out, double coef, int accumulate) {
+                                                double *__restrict__ out, double coef, int accumulate,
+                                                int64_t slot_stride) {
     const int gi = gorder[blockIdx.x];
     const FarGroup G = groups[gi];
     if (active && !active[G.susp]) return;
@@ -247,7 +248,7 @@ __global__ __launch_bounds__(64) void k_far_f64(const double *__restrict__ src, 
         else
             far_range<KIND, TPL, true>(src, b, e, x, y, z, nx, ny, nz, incl, ax, ay, az);
     }
-    store_far<TPL>(G, lane, ax, ay, az, out, coef, accumulate);
+    store_far<TPL>(G, lane, ax, ay, az, out, coef, accumulate, slot_stride);
 }
 
 // far field, FP32 pair arithmetic with two-level accumulation: FP32 partial sums
@@ -258,7 +259,8 @@ template <int KIND, int TPL>
 __global__ __launch_bounds__(64) void k_far_f32(const float *__restrict__ src, const float *__restrict__ trg,
                                                 const int *__restrict__ gorder, const FarGroup *__restrict__ groups,
                                                 const FarInterval *__restrict__ iv, const int *__restrict__ active,
-                                                double *__restrict__ out, double coef, int accumulate) {
+                                                double *__restrict__ out, double coef, int accumulate,
+                                                int64_t slot_stride) {
     const int gi = gorder[blockIdx.x];
     const FarGroup G = groups[gi];
     if (active && !active[G.susp]) return;
@@ -304,41 +306,50 @@ __global__ __launch_bounds__(64) void k_far_f32(const float *__restrict__ src, c
             }
         }
     }
-    store_far<TPL>(G, lane, ax, ay, az, out, coef, accumulate);
+    store_far<TPL>(G, lane, ax, ay, az, out, coef, accumulate, slot_stride);
 }
 
+struct FarLaunch {
+    const void *src;
+    const double *tcoords, *tnorms;
+    const float *trg32;
+    int64_t ntrg;
+};
+
 template <int KIND, int TPL>
-static void far_dispatch_prec(int prec, const DevGeom &g, const void *src, const float *trg32, int ngroups,
-                              const int *go, const FarGroup *gr, const FarInterval *iv, const int *active,
-                              double *out, double coef, int acc, hipStream_t s) {
+static void far_dispatch_prec(int prec, const FarLaunch &a, int ngroups, const int *go, const FarGroup *gr,
+                              const FarInterval *iv, const int *active, double *out, double coef, int acc,
+                              hipStream_t s) {
+    const int64_t stride = 3 * a.ntrg;
     if (prec == 0)
-        k_far_f64<KIND, TPL><<<dim3(ngroups), dim3(64), 0, s>>>((const double *)src, g.coords, g.norms, go, gr, iv,
-                                                               active, out, coef, acc);
+        k_far_f64<KIND, TPL><<<dim3(ngroups), dim3(64), 0, s>>>((const double *)a.src, a.tcoords, a.tnorms, go, gr,
+                                                               iv, active, out, coef, acc, stride);
     else
-        k_far_f32<KIND, TPL><<<dim3(ngroups), dim3(64), 0, s>>>((const float *)src, trg32, go, gr, iv, active, out,
-                                                               coef, acc);
+        k_far_f32<KIND, TPL><<<dim3(ngroups), dim3(64), 0, s>>>((const float *)a.src, a.trg32, go, gr, iv, active,
+                                                               out, coef, acc, stride);
 }
 
 template <int KIND>
-static void far_dispatch_tpl(int tpl, int prec, const DevGeom &g, const void *src, const float *trg32, int ngroups,
-                             const int *go, const FarGroup *gr, const FarInterval *iv, const int *active,
-                             double *out, double coef, int acc, hipStream_t s) {
+static void far_dispatch_tpl(int tpl, int prec, const FarLaunch &a, int ngroups, const int *go, const FarGroup *gr,
+                             const FarInterval *iv, const int *active, double *out, double coef, int acc,
+                             hipStream_t s) {
     switch (tpl) {
-    case 1: far_dispatch_prec<KIND, 1>(prec, g, src, trg32, ngroups, go, gr, iv, active, out, coef, acc, s); break;
-    case 2: far_dispatch_prec<KIND, 2>(prec, g, src, trg32, ngroups, go, gr, iv, active, out, coef, acc, s); break;
-    case 4: far_dispatch_prec<KIND, 4>(prec, g, src, trg32, ngroups, go, gr, iv, active, out, coef, acc, s); break;
+    case 1: far_dispatch_prec<KIND, 1>(prec, a, ngroups, go, gr, iv, active, out, coef, acc, s); break;
+    case 2: far_dispatch_prec<KIND, 2>(prec, a, ngroups, go, gr, iv, active, out, coef, acc, s); break;
+    case 4: far_dispatch_prec<KIND, 4>(prec, a, ngroups, go, gr, iv, active, out, coef, acc, s); break;
     default: throw Error(-1, "unsupported targets-per-lane");
     }
 }
 
-void launch_far(int kind, int prec, int tpl, const DevGeom &g, const void *src, const float *trg32, int ngroups,
-                const int *go, const FarGroup *gr, const FarInterval *iv, const int *active, double *out,
-                double coef, int acc, hipStream_t s) {
+void launch_far(int kind, int prec, int tpl, const void *src, const double *tcoords, const double *tnorms,
+                const float *trg32, int64_t ntrg, int ngroups, const int *go, const FarGroup *gr,
+                const FarInterval *iv, const int *active, double *out, double coef, int acc, hipStream_t s) {
     if (ngroups <= 0) return;
+    const FarLaunch a{src, tcoords, tnorms, trg32, ntrg};
     if (kind == 0)
-        far_dispatch_tpl<0>(tpl, prec, g, src, trg32, ngroups, go, gr, iv, active, out, coef, acc, s);
+        far_dispatch_tpl<0>(tpl, prec, a, ngroups, go, gr, iv, active, out, coef, acc, s);
     else
-        far_dispatch_tpl<1>(tpl, prec, g, src, trg32, ngroups, go, gr, iv, active, out, coef, acc, s);
+        far_dispatch_tpl<1>(tpl, prec, a, ngroups, go, gr, iv, active, out, coef, acc, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -378,6 +389,67 @@ __global__ __launch_bounds__(256) void k_analysis(DevGeom g, const double *__res
         for (; c < cols; c++) a0 = __builtin_fma(A[(int64_t)c * rows + r], vb[c], a0);
         cS[r] = (a0 + a1) + (a2 + a3);
     }
+}
+
+// Per-sphere resampling between two point sets of the sphere (fine grid and
+// coarse grid): out_S (+)= R_S M_p R_S^-1 in_S with M_p column-major
+// (3 nout x 3 nin) acting on body-frame values.
+__global__ __launch_bounds__(256) void k_resample(DevGeom g, ResampleTabs tab, const int64_t *__restrict__ in_off,
+                                                  const int64_t *__restrict__ out_off, const double *__restrict__ in,
+                                                  double *__restrict__ out, int accumulate,
+                                                  const int *__restrict__ active) {
+    extern __shared__ double lds_rs[];
+    const int S = blockIdx.x;
+    if (active && !active[g.sph_susp[S]]) return;
+    const int p = g.order[S];
+    const double *__restrict__ M = tab.M[p];
+    const int nin = tab.nin[p], nout = tab.nout[p];
+    const int64_t io = in_off[S], oo = out_off[S];
+    const double *q = g.quat + 4 * S;
+    double *vin = lds_rs, *vout = lds_rs + 3 * nin;
+    for (int i = threadIdx.x; i < nin; i += blockDim.x) {
+        const double v[3] = {in[3 * (io + i)], in[3 * (io + i) + 1], in[3 * (io + i) + 2]};
+        double o[3];
+        qrot(q, v, o, true);
+        vin[3 * i] = o[0];
+        vin[3 * i + 1] = o[1];
+        vin[3 * i + 2] = o[2];
+    }
+    __syncthreads();
+    const int rows = 3 * nout, cols = 3 * nin;
+    for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+        double a0 = 0, a1 = 0;
+        int c = 0;
+        for (; c + 1 < cols; c += 2) {
+            a0 = __builtin_fma(M[(int64_t)c * rows + r], vin[c], a0);
+            a1 = __builtin_fma(M[(int64_t)(c + 1) * rows + r], vin[c + 1], a1);
+        }
+        if (c < cols) a0 = __builtin_fma(M[(int64_t)c * rows + r], vin[c], a0);
+        vout[r] = a0 + a1;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nout; i += blockDim.x) {
+        double o[3];
+        qrot(q, vout + 3 * i, o, false);
+        double *d = out + 3 * (oo + i);
+        if (accumulate) {
+            d[0] += o[0];
+            d[1] += o[1];
+            d[2] += o[2];
+        } else {
+            d[0] = o[0];
+            d[1] = o[1];
+            d[2] = o[2];
+        }
+    }
+}
+
+void launch_resample(const DevGeom &g, const ResampleTabs &tab, const int64_t *in_off, const int64_t *out_off,
+                     const double *in, double *out, int accumulate, const int *active, hipStream_t s,
+                     int max_points) {
+    if (!g.nsph) return;
+    const size_t lds = sizeof(double) * 3 * 2 * (size_t)max_points;
+    k_resample<<<dim3(g.nsph), dim3(256), lds, s>>>(g, tab, in_off, out_off, in, out, accumulate, active);
 }
 
 void launch_analysis(const DevGeom &g, const double *in, double *coef, const int *active, hipStream_t s,
@@ -824,6 +896,24 @@ __global__ __launch_bounds__(256) void k_seg_multi_axpy(double *__restrict__ x, 
         for (int j = 0; j < k; j++) s = __builtin_fma(coef[(int64_t)j * nseg + c.seg], V[(int64_t)j * ldv + i], s);
         x[i] = s;
     }
+}
+
+// out[i] += sum_k part[k*stride + i], k = 0..nslot-1, over the DOF chunks of active segments
+__global__ __launch_bounds__(256) void k_slot_reduce(double *__restrict__ out, const double *__restrict__ part,
+                                                     int nslot, int64_t stride, const Chunk *__restrict__ ch,
+                                                     const int *__restrict__ active) {
+    const Chunk c = ch[blockIdx.x];
+    if (active && !active[c.seg]) return;
+    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += blockDim.x) {
+        double s = out[i];
+        for (int k = 0; k < nslot; k++) s += part[k * stride + i];
+        out[i] = s;
+    }
+}
+
+void launch_slot_reduce(double *out, const double *part, int nslot, int64_t stride, const Chunk *chunks,
+                        int nchunks, const int *active, hipStream_t s) {
+    if (nchunks > 0) k_slot_reduce<<<dim3(nchunks), dim3(256), 0, s>>>(out, part, nslot, stride, chunks, active);
 }
 
 __global__ void k_axpby(int64_t n, double a, const double *__restrict__ x, double b, double *__restrict__ y) {

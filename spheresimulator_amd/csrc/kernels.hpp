@@ -33,14 +33,31 @@ struct Chunk {
     int32_t seg, pad;
 };
 
-void launch_pack_src_f64(const DevGeom &g, const double *in, double *src, hipStream_t s);
-void launch_pack_src_f32(const DevGeom &g, const double *in, float *src, hipStream_t s);
-void launch_pack_trg_f32(const DevGeom &g, float *trg, hipStream_t s);
+// Per-order resampling matrices on device (see Resample in common.hpp).
+struct ResampleTabs {
+    const double *M[kMaxOrder + 1] = {};
+    int nin[kMaxOrder + 1] = {};
+    int nout[kMaxOrder + 1] = {};
+};
 
-// far field: out = (accumulate ? out : 0) + coef * sum over far pairs
-void launch_far(int kind, int prec, int tpl, const DevGeom &g, const void *src, const float *trg32, int ngroups,
-                const int *group_order, const FarGroup *groups, const FarInterval *iv, const int *active,
-                double *out, double coef, int accumulate, hipStream_t s);
+// source records (x, y, z, w*in, 0, 0) and FP32 target records (x, n, 0, 0) of n points
+void launch_pack_src_f64(int64_t n, const double *coords, const double *w, const double *in, double *src,
+                         hipStream_t s);
+void launch_pack_src_f32(int64_t n, const double *coords, const double *w, const double *in, float *src,
+                         hipStream_t s);
+void launch_pack_trg_f32(int64_t n, const double *coords, const double *norms, float *trg, hipStream_t s);
+
+// far field over ntrg targets (FP64: tcoords/tnorms, FP32: trg32 records):
+// accumulate=1: out += coef * sum over the listed pairs;
+// accumulate=0: slot k of out (stride 3*ntrg) = coef * the sum over source slice k
+void launch_far(int kind, int prec, int tpl, const void *src, const double *tcoords, const double *tnorms,
+                const float *trg32, int64_t ntrg, int ngroups, const int *group_order, const FarGroup *groups,
+                const FarInterval *iv, const int *active, double *out, double coef, int accumulate, hipStream_t s);
+
+// out_S (+)= R_S M_p R_S^-1 in_S between two point sets with per-sphere offsets
+void launch_resample(const DevGeom &g, const ResampleTabs &tab, const int64_t *in_off, const int64_t *out_off,
+                     const double *in, double *out, int accumulate, const int *active, hipStream_t s,
+                     int max_points);
 
 void launch_analysis(const DevGeom &g, const double *in, double *coef, const int *active, hipStream_t s,
                      int max_order);
@@ -71,6 +88,9 @@ void launch_seg_scale(double *y, const double *x, const double *alpha, const Chu
 // x += sum_i coef[i*nseg + seg] * V[i]   (i < k)
 void launch_seg_multi_axpy(double *x, const double *V, int64_t ldv, const double *coef, int k, int nseg,
                            const Chunk *chunks, int nchunks, const int *active, hipStream_t s);
+// out += sum of nslot partial-sum slots (far-field source slices)
+void launch_slot_reduce(double *out, const double *part, int nslot, int64_t stride, const Chunk *chunks,
+                        int nchunks, const int *active, hipStream_t s);
 // y = a*x + b*y (plain, whole vector)
 void launch_axpby(int64_t n, double a, const double *x, double b, double *y, hipStream_t s);
 void launch_fill(int64_t n, double v, double *y, hipStream_t s);

@@ -12,6 +12,7 @@
 // quadrature inner product, which is exact for vector-SH degree <= p.
 #include "common.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <map>
@@ -107,6 +108,7 @@ static std::unique_ptr<ShTables> build_tables(int p) {
             for (int k = 2; k <= n + m; k++) f *= k;
             invN[tri(n, m)] = 1.0 / f;
         }
+    t->synth.assign(size_t(cols) * rows, 0.0);
     std::vector<cplx> D;
     for (int q = 0; q < t->npts; q++) {
         const double *e = &t->eta[3 * q];
@@ -130,6 +132,18 @@ static std::unique_ptr<ShTables> build_tables(int p) {
                 xt[0] = e[1] * g[2] - e[2] * g[1];
                 xt[1] = e[2] * g[0] - e[0] * g[2];
                 xt[2] = e[0] * g[1] - e[1] * g[0];
+                // synthesis: value_c = sum_k eps_m Re(F_r y e_c + F_s gs_c + F_t xt_c), F stored (re, im)
+                const double eps = m ? 2.0 : 1.0;
+                for (int c = 0; c < 3; c++) {
+                    const size_t r = size_t(3 * q + c);
+                    const cplx br = y * e[c], bs = gs[c], bt = xt[c];
+                    t->synth[size_t(2 * k) * cols + r] = eps * br.real();
+                    t->synth[size_t(2 * k + 1) * cols + r] = -eps * br.imag();
+                    t->synth[size_t(2 * Q + 2 * k) * cols + r] = eps * bs.real();
+                    t->synth[size_t(2 * Q + 2 * k + 1) * cols + r] = -eps * bs.imag();
+                    t->synth[size_t(4 * Q + 2 * k) * cols + r] = eps * bt.real();
+                    t->synth[size_t(4 * Q + 2 * k + 1) * cols + r] = -eps * bt.imag();
+                }
                 const double wr = t->w[q] * invN[k];
                 const double wt = n > 0 ? t->w[q] * invN[k] / (n * (n + 1.0)) : 0.0;
                 for (int c = 0; c < 3; c++) {
@@ -149,6 +163,65 @@ static std::unique_ptr<ShTables> build_tables(int p) {
             }
     }
     return t;
+}
+
+// rows of a coefficient vector (6Q layout) that belong to degrees <= pc
+static std::vector<int> coef_rows_upto(int p, int pc) {
+    const int Q = tri_size(p);
+    std::vector<int> r;
+    for (int blk = 0; blk < 3; blk++)
+        for (int n = 0; n <= std::min(p, pc); n++)
+            for (int m = 0; m <= n; m++) {
+                r.push_back(blk * 2 * Q + 2 * tri(n, m));
+                r.push_back(blk * 2 * Q + 2 * tri(n, m) + 1);
+            }
+    return r;
+}
+
+static std::unique_ptr<Resample> build_resample(int p) {
+    auto rs = std::make_unique<Resample>();
+    rs->p = p;
+    rs->pc = coarse_order(p);
+    const ShTables &F = sh_tables(p), &C = sh_tables(rs->pc);
+    rs->n = F.npts;
+    rs->nc = C.npts;
+    const int QF = F.Q, QC = C.Q;
+    const int rowsF = 6 * QF, rowsC = 6 * QC;
+    const std::vector<int> rf = coef_rows_upto(p, rs->pc), rc = coef_rows_upto(rs->pc, rs->pc);
+    // down = synth_C[:, rc] * anal_F[rf, :]
+    const int nF = 3 * F.npts, nC = 3 * C.npts;
+    rs->down.assign(size_t(nC) * nF, 0.0);
+    for (int col = 0; col < nF; col++)
+        for (size_t k = 0; k < rf.size(); k++) {
+            const double a = F.anal[size_t(col) * rowsF + rf[k]];
+            if (a == 0.0) continue;
+            const double *sc = &C.synth[size_t(rc[k]) * nC];
+            double *o = &rs->down[size_t(col) * nC];
+            for (int r = 0; r < nC; r++) o[r] += sc[r] * a;
+        }
+    // up = synth_F[:, rf] * anal_C[rc, :]
+    rs->up.assign(size_t(nF) * nC, 0.0);
+    for (int col = 0; col < nC; col++)
+        for (size_t k = 0; k < rc.size(); k++) {
+            const double a = C.anal[size_t(col) * rowsC + rc[k]];
+            if (a == 0.0) continue;
+            const double *sf = &F.synth[size_t(rf[k]) * nF];
+            double *o = &rs->up[size_t(col) * nF];
+            for (int r = 0; r < nF; r++) o[r] += sf[r] * a;
+        }
+    return rs;
+}
+
+const Resample &resample_tables(int p) {
+    static std::mutex mu;
+    static std::map<int, std::unique_ptr<Resample>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(p);
+    if (it != cache.end()) return *it->second;
+    auto t = build_resample(p);
+    const Resample &ref = *t;
+    cache[p] = std::move(t);
+    return ref;
 }
 
 const ShTables &sh_tables(int p) {

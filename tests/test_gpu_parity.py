@@ -199,3 +199,21 @@ def test_warm_start_and_density():
     vr, _ = ref.solve(F, tol=1e-8)
     assert rel(v1, vr) < 1e-10
     assert np.isfinite(rho).all() and rho.size == gpu.op.n_dof
+
+
+@pytest.mark.parametrize("p,direct,tol", [(8, True, 2e-6), (8, False, 5e-5), (12, False, 1e-6)])
+def test_inner_fp32_operator_accuracy(p, direct, tol):
+    """The FP32 inner operator approximates the FP64 operator on a white band-
+    limited density (the projection of a random field).  Direct FP32 far field:
+    FP32 rounding only.  Two-level far field (pairs beyond 4 (R_s + R_t) through
+    coarse grids of order p/2+1): the truncated degrees decay like (1/8)^(p/2+2)."""
+    say("inner FP32 operator (p=%d direct=%s)" % (p, direct))
+    c = S.configs.make_suspension(96 if p == 8 else 256, p, 31, vf=0.1, min_gap=0.2)
+    gpu = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0,
+                                fp32_direct=direct)
+    x = gpu.projectNullSpace(np.random.default_rng(8).standard_normal(gpu.n_dof))
+    y64 = gpu.apply(x)
+    y32 = gpu.apply_inner(x)
+    err = rel(y32, y64)
+    print("inner rel err %.3e" % err)
+    assert err < tol
