@@ -52,6 +52,16 @@ def run_form(S, c, precision, steps, warmup):
     return M, v, info, times, stats
 
 
+def pmc_traffic(c, pairs):
+    """HBM bytes per launch of the FP64 traction kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE passes (profiles/), when they cover this workload."""
+    path = os.path.join(ROOT, "profiles", "far_traction_f64_cfg2_pmc.json")
+    if not os.path.exists(path) or not c["name"].startswith("cfg2"):
+        return None
+    with open(path) as f:
+        return json.load(f)["traffic_bytes"]
+
+
 def cpu_baseline(c, iters, budget_s=15.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
@@ -121,7 +131,7 @@ def main():
             ach = FLOPS_TRAC * st["pairs"] / avg_s / 1e12
             out["roofline"] = {"bound": "valu", "kernel": "k_far_f64<traction>", "achieved": ach,
                                "peak": FP64_VALU_PEAK, "unit": "TFLOP/s", "frac": ach / FP64_VALU_PEAK,
-                               "traffic": None, "avg_launch_ms": st["ms"] / st["launches"],
+                               "traffic": pmc_traffic(c, st["pairs"]), "avg_launch_ms": st["ms"] / st["launches"],
                                "pairs_per_launch": st["pairs"], "flops_per_pair": FLOPS_TRAC}
     if "mixed" in res and "fp64" in res:
         vm = res["mixed"]["v"].reshape(-1, 6)
