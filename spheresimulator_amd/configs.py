@@ -91,8 +91,9 @@ class _Grid:
         self.rad.append(float(r))
 
 
-def _rsa(st, radii, lo, hi, gap_fn, max_tries=2000000):
-    """Random sequential addition of spheres of the given radii in the box [lo, hi]^3."""
+def _rsa(st, radii, lo, hi, gap_fn, max_tries=2000000, wall=True):
+    """Random sequential addition of spheres of the given radii with centres in
+    the cube [lo, hi]^3 (whole spheres inside it when `wall`)."""
     n = len(radii)
     h = 2 * max(radii) + gap_fn(max(radii), max(radii)) + 1e-9
     g = _Grid(h)
@@ -106,7 +107,7 @@ def _rsa(st, radii, lo, hi, gap_fn, max_tries=2000000):
             if bi == batch:
                 buf = st.uniform(3 * batch).reshape(batch, 3)
                 bi = 0
-            x = lo + r + buf[bi] * (hi - lo - 2 * r)
+            x = lo + r + buf[bi] * (hi - lo - 2 * r) if wall else lo + buf[bi] * (hi - lo)
             bi += 1
             tries += 1
             if tries > max_tries:
@@ -170,7 +171,7 @@ def config4(seed=4, n_susp=256):
         vf = 0.05 + 0.10 * st.uniform(1)[0]
         r = 0.5 + st.uniform(n)
         box = (np.sum(4.0 / 3.0 * math.pi * r ** 3) / vf) ** (1.0 / 3.0)
-        pb = _rsa(st, r, 0.0, box, lambda a, c: 0.1 * (a + c))
+        pb = _rsa(st, r, 0.0, box, lambda a, c: 0.1 * (a + c), wall=False)
         pos.append(pb)
         rad.append(r)
         order.append(np.full(n, p, dtype=np.int32))

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
@@ -685,6 +686,7 @@ Mobility::Mobility(int nsusp, const int *counts, const double *pos, const double
     : mu_(viscosity), prm_(prm), prec_(precision) {
     SPHSTK_REQUIRE(std::isfinite(viscosity) && viscosity > 0, "viscosity must be > 0");
     SPHSTK_REQUIRE(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (mixed)");
+    if (const char *v = std::getenv("SPHSTK_VERBOSE")) verbose_ = std::atoi(v) != 0;
     A_ = std::make_unique<Operator>(nsusp, counts, pos, radius, quat, order, 0.5, 0.0, 1.0, 1.0, flags);
     const int64_t n = std::max<int64_t>(A_->ndof(), 1);
     rho_.alloc(n);
@@ -829,6 +831,18 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
                 if (info[s].relres <= tol || hn == 0.0) cur[s] = 0;
             }
             total++;
+            if (verbose_) {
+                double worst = 0.0;
+                int nact = 0;
+                for (int s = 0; s < B; s++)
+                    if (cyc[s]) {
+                        worst = std::max(worst, info[s].relres);
+                        nact += cur[s];
+                    }
+                std::fprintf(stderr, "[sphstk] gmres(%s) iter %d: max implicit relres %.3e, %d/%d suspensions active\n",
+                             prec ? "fp32 inner" : "fp64", total, worst, nact, B);
+                std::fflush(stderr);
+            }
             if (!any(cur) || total >= prm_.max_iters) break;
             upload_act(cur);
         }

@@ -201,6 +201,7 @@ class Mobility {
     SolverParams prm_;
     int prec_;
     bool x_zero_ = true; // the stored unknown is known to be zero (cold start)
+    bool verbose_ = false; // SPHSTK_VERBOSE=1: residual history on stderr (Belos Verbosity 63)
     DevBuf<double> rho_, b_, x_, r_, d_, V_, h_, scal_, coefv_, force_, vel_, gmres_inner_rhs_;
     DevBuf<int> act_;
 };
