@@ -73,19 +73,16 @@ struct ShTables {
 
 const ShTables &sh_tables(int p);
 
-// Order of the coarse grid used by the two-level far field of the FP32 inner operator.
-inline int coarse_order(int p) { return p <= 4 ? p : p / 2 + 1; }
-
-// Per-order resampling matrices between the order-p grid and the coarse grid of
-// order pc = coarse_order(p), both acting on body-frame values (3 per point),
-// column-major:
+// Per-order resampling matrices between the order-p grid and a coarser grid of
+// order pc <= p, both acting on body-frame values (3 per point), column-major:
 //   down (3 nc x 3 n): coarse values of the degree-pc truncation of the projection
 //   up   (3 n x 3 nc): fine values of the degree-pc projection of coarse values
+// (pc == p: identity in both directions).
 struct Resample {
     int p = 0, pc = 0, n = 0, nc = 0;
     std::vector<double> down, up;
 };
-const Resample &resample_tables(int p);
+const Resample &resample_tables(int p, int pc);
 void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w);
 
 } // namespace sphstk

@@ -159,6 +159,35 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     near_src_.reserve(near_off_[nsph_]);
     for (int t = 0; t < nsph_; t++) near_src_.insert(near_src_.end(), nb[t].begin(), nb[t].end());
 
+    // self/near evaluation work items: per target sphere its own expansion first,
+    // then its near sources in list order, each cut into chunks of 64 targets
+    {
+        std::vector<EvalItem> items, selfi;
+        std::vector<int64_t> tso(nsph_ + 1, 0);
+        std::vector<int> tns(nsph_, 0);
+        int64_t slot = 0;
+        for (int t = 0; t < nsph_; t++) {
+            const int nT = (int)(pt_off_[t + 1] - pt_off_[t]);
+            tso[t] = slot;
+            tns[t] = 1 + (int)(near_off_[t + 1] - near_off_[t]);
+            for (int k = 0; k < tns[t]; k++) {
+                const int s = (k == 0) ? t : near_src_[near_off_[t] + k - 1];
+                for (int j = 0; j < nT; j += 64) {
+                    EvalItem it{t, s, j, std::min(64, nT - j), slot + 3 * (int64_t)j};
+                    items.push_back(it);
+                    if (k == 0) selfi.push_back(it);
+                }
+                slot += 3 * (int64_t)nT;
+            }
+        }
+        tso[nsph_] = slot;
+        items_.upload(items);
+        self_items_.upload(selfi);
+        d_trg_slot_off_.upload(tso);
+        d_trg_nsrc_.upload(tns);
+        evscratch_.alloc(std::max<int64_t>(slot, 1));
+    }
+
     // device state
     SPHSTK_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     d_pos_.upload(pos_);
@@ -209,8 +238,8 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     if (const char *v = std::getenv("SPHSTK_FAR_TPL")) tpl = std::atoi(v);
     if (const char *v = std::getenv("SPHSTK_FAR_SPLIT")) split = std::atoi(v);
     tpl_ = tpl;
-    two_level_ = (flags_ & 2) == 0;
-    if (const char *v = std::getenv("SPHSTK_TWO_LEVEL")) two_level_ = std::atoi(v) != 0;
+    multi_level_ = (flags_ & 2) == 0;
+    if (const char *v = std::getenv("SPHSTK_MULTI_LEVEL")) multi_level_ = std::atoi(v) != 0;
     {
         std::vector<std::vector<int>> excl(nsph_);
         for (int t = 0; t < nsph_; t++) {
@@ -398,61 +427,81 @@ static std::vector<Chunk> make_chunks(int nsusp, const std::vector<int64_t> &sus
     return ch;
 }
 
-// Two-level far field of the FP32 inner operator: pairs of spheres closer than
-// alpha (R_s + R_t) (and not near) interact point to point on the fine grids;
-// all other far pairs interact between the coarse grids (order coarse_order(p))
-// through the degree-truncated density, and the coarse-grid field is
-// interpolated back to the fine grid.
-void Operator::ensure_two_level() {
-    if (two_level_ready_) return;
-    double alpha = 4.0;
-    if (const char *v = std::getenv("SPHSTK_TWO_LEVEL_ALPHA")) alpha = std::atof(v);
-    SPHSTK_REQUIRE(alpha >= 1.5, "two-level separation factor must be >= 1.5");
-    alpha_ = alpha;
-    // coarse geometry
-    pc_off_.assign(nsph_ + 1, 0);
-    int maxpts = 0;
-    for (int i = 0; i < nsph_; i++) {
-        const Resample &rs = resample_tables(order_[i]);
-        pc_off_[i + 1] = pc_off_[i] + rs.nc;
-        maxpts = std::max(maxpts, std::max(rs.n, rs.nc));
-    }
-    max_rs_points_ = maxpts;
-    const int64_t nc = pc_off_[nsph_];
-    std::vector<double> cc(3 * nc), cn(3 * nc), cw(nc);
-    for (int i = 0; i < nsph_; i++) {
-        const ShTables &t = sh_tables(coarse_order(order_[i]));
-        for (int q = 0; q < t.npts; q++) {
-            const int64_t g = pc_off_[i] + q;
-            double nv[3];
-            qrot_host(&quat_[4 * i], &t.eta[3 * q], nv);
+// Multi-level far field of the FP32 inner operator.  With s = |c_S - c_T| /
+// (R_S + R_T), a far pair (s >= 1.5) interacts
+//   s < 4        point to point on the fine grids ("mid"),
+//   4 <= s < 8   between grids of order p/2+1            (level 0),
+//   8 <= s < 14  between grids of order max(5, p/3+1)    (level 1),
+//   s >= 14      between grids of order 4                (level 2),
+// (orders capped by p) through the degree-truncated density of the source
+// (Resample::down) with the target-grid field interpolated back to the fine grid
+// (Resample::up).  The truncated degrees of each level contribute below ~1e-6
+// relative: the field of degree-n density decays like (1/(2s))^(n+2) and its
+// angular content on the target like (1/(2s-1))^(n+1).  Level point sets are
+// stored in Morton order of the sphere centres (per suspension) so that the
+// spatially compact exclusion sets of the last level are few index ranges.
+namespace {
+struct LevelSpec {
+    double alo, ahi;
+};
+constexpr int kNumLevels = 3;
+int level_order(int lvl, int p) {
+    int pc = p;
+    if (lvl == 0) pc = p < 10 ? p : p / 2 + 1;
+    if (lvl == 1) pc = std::max(5, p / 3 + 1);
+    if (lvl == 2) pc = 4;
+    return std::min(pc, p);
+}
+uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+    auto spread = [](uint64_t v) {
+        v &= 0x1fffff;
+        v = (v | v << 32) & 0x1f00000000ffffULL;
+        v = (v | v << 16) & 0x1f0000ff0000ffULL;
+        v = (v | v << 8) & 0x100f00f00f00f00fULL;
+        v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+        v = (v | v << 2) & 0x1249249249249249ULL;
+        return v;
+    };
+    return spread(x) | spread(y) << 1 | spread(z) << 2;
+}
+} // namespace
+
+void Operator::ensure_levels() {
+    if (levels_ready_) return;
+    // separations (in units of R_s + R_t) where levels 0, 1, 2 start
+    double a[kNumLevels] = {4.0, 8.0, 14.0};
+    if (const char *v = std::getenv("SPHSTK_LEVEL_ALPHA"))
+        if (std::sscanf(v, "%lf,%lf,%lf", &a[0], &a[1], &a[2]) != 3) throw Error(-1, "SPHSTK_LEVEL_ALPHA: a0,a1,a2");
+    SPHSTK_REQUIRE(a[0] >= 1.5 && a[1] > a[0] && a[2] > a[1], "level separations must increase from >= 1.5");
+    const LevelSpec kLevels[kNumLevels] = {{a[0], a[1]}, {a[1], a[2]}, {a[2], 1e300}};
+    // Morton order of the spheres within each suspension
+    std::vector<int> perm(nsph_), rank(nsph_);
+    for (int b = 0; b < nsusp_; b++) {
+        const int s0 = (int)susp_sph_off_[b], s1 = (int)susp_sph_off_[b + 1];
+        if (s1 <= s0) continue;
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int i = s0; i < s1; i++)
             for (int c = 0; c < 3; c++) {
-                cn[3 * g + c] = nv[c];
-                cc[3 * g + c] = pos_[3 * i + c] + rad_[i] * nv[c];
+                lo[c] = std::min(lo[c], pos_[3 * i + c]);
+                hi[c] = std::max(hi[c], pos_[3 * i + c]);
             }
-            cw[g] = rad_[i] * rad_[i] * t.w[q];
+        const double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
+        std::vector<std::pair<uint64_t, int>> key;
+        for (int i = s0; i < s1; i++) {
+            uint32_t q[3];
+            for (int c = 0; c < 3; c++) q[c] = (uint32_t)std::min(2097151.0, (pos_[3 * i + c] - lo[c]) / ext * 2097151.0);
+            key.push_back({morton3(q[0], q[1], q[2]), i});
+        }
+        std::sort(key.begin(), key.end());
+        for (int k = 0; k < s1 - s0; k++) {
+            perm[s0 + k] = key[k].second;
+            rank[key[k].second] = s0 + k;
         }
     }
-    d_pc_off_.upload(pc_off_);
-    d_ccoords_.upload(cc);
-    d_cnorms_.upload(cn);
-    d_cw_.upload(cw);
-    std::vector<bool> used(kMaxOrder + 1, false);
-    for (int p : order_) used[p] = true;
-    for (int p = 0; p <= kMaxOrder; p++)
-        if (used[p]) {
-            const Resample &rs = resample_tables(p);
-            d_down_[p].upload(rs.down);
-            d_up_[p].upload(rs.up);
-            down_tab_.M[p] = d_down_[p].p;
-            down_tab_.nin[p] = rs.n;
-            down_tab_.nout[p] = rs.nc;
-            up_tab_.M[p] = d_up_[p].p;
-            up_tab_.nin[p] = rs.nc;
-            up_tab_.nout[p] = rs.n;
-        }
-    // mid lists: not near, not self, centre distance < alpha (R_s + R_t), same suspension
-    std::vector<std::vector<int>> mid(nsph_), excl(nsph_);
+    // neighbour classes: mid (fine), shells of levels 0..L-2, exclusion list of the last level
+    const double amax = kLevels[kNumLevels - 1].alo;
+    std::vector<std::vector<int>> mid(nsph_);
+    std::vector<std::vector<std::vector<int>>> shell(kNumLevels, std::vector<std::vector<int>>(nsph_));
     for (int b = 0; b < nsusp_; b++) {
         const int s0 = (int)susp_sph_off_[b], s1 = (int)susp_sph_off_[b + 1];
         if (s1 <= s0) continue;
@@ -461,7 +510,7 @@ void Operator::ensure_two_level() {
             rmax = std::max(rmax, rad_[i]);
             for (int c = 0; c < 3; c++) lo[c] = std::min(lo[c], pos_[3 * i + c]);
         }
-        const double h = 2.0 * alpha * rmax;
+        const double h = 2.0 * amax * rmax;
         std::unordered_map<int64_t, std::vector<int>> cells;
         auto key = [&](int64_t a, int64_t bb, int64_t c) { return (a * 1000003LL + bb) * 1000033LL + c; };
         std::vector<std::array<int64_t, 3>> cid(s1 - s0);
@@ -477,31 +526,83 @@ void Operator::ensure_two_level() {
                         auto it = cells.find(key(ct[0] + dx, ct[1] + dy, ct[2] + dz));
                         if (it == cells.end()) continue;
                         for (int s : it->second) {
-                            if (s == t) continue;
                             const double ddx = pos_[3 * s] - pos_[3 * t], ddy = pos_[3 * s + 1] - pos_[3 * t + 1],
                                          ddz = pos_[3 * s + 2] - pos_[3 * t + 2];
-                            const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                            const double rn = 1.5 * (rad_[s] + rad_[t]), rm = alpha * (rad_[s] + rad_[t]);
-                            if (d2 >= rn * rn && d2 < rm * rm) mid[t].push_back(s);
+                            const double sep = std::sqrt(ddx * ddx + ddy * ddy + ddz * ddz) / (rad_[s] + rad_[t]);
+                            if (s == t || sep < amax) shell[kNumLevels - 1][rank[t]].push_back(rank[s]);
+                            if (s == t || sep < 1.5) continue;
+                            if (sep < kLevels[0].alo) {
+                                mid[t].push_back(s);
+                                continue;
+                            }
+                            for (int l = 0; l + 1 < kNumLevels; l++)
+                                if (sep >= kLevels[l].alo && sep < kLevels[l].ahi) shell[l][rank[t]].push_back(rank[s]);
                         }
                     }
             std::sort(mid[t].begin(), mid[t].end());
-            excl[t].push_back(t);
-            for (int64_t k = near_off_[t]; k < near_off_[t + 1]; k++) excl[t].push_back(near_src_[k]);
-            excl[t].insert(excl[t].end(), mid[t].begin(), mid[t].end());
-            std::sort(excl[t].begin(), excl[t].end());
+            for (int l = 0; l < kNumLevels; l++) std::sort(shell[l][rank[t]].begin(), shell[l][rank[t]].end());
         }
     }
     build_far_lists(lists_mid_, nsusp_, susp_sph_off_, pt_off_, pt_off_, mid, true, tpl_, 0, kWantItems);
-    build_far_lists(lists_coarse_, nsusp_, susp_sph_off_, pc_off_, pc_off_, excl, false, tpl_, 0, kWantItems);
-    std::vector<int> dummy;
-    cchunks_.upload(make_chunks(nsusp_, susp_sph_off_, pc_off_, &dummy));
-    sc_.alloc(std::max<int64_t>(3 * nc, 1));
-    tc_.alloc(std::max<int64_t>(3 * nc, 1));
-    src32c_.alloc(std::max<int64_t>(8 * nc, 8));
-    trg32c_.alloc(std::max<int64_t>(8 * nc, 8));
-    launch_pack_trg_f32(nc, d_ccoords_.p, d_cnorms_.p, trg32c_.p, stream_);
-    two_level_ready_ = true;
+    levels_.clear();
+    for (int l = 0; l < kNumLevels; l++) {
+        auto L = std::make_unique<FarLevel>();
+        std::vector<int64_t> off(nsph_ + 1, 0), off_by_sph(nsph_);
+        int maxpts = 0;
+        for (int r = 0; r < nsph_; r++) {
+            const int s = perm[r];
+            const int p = order_[s], pc = level_order(l, p);
+            off[r + 1] = off[r] + grid_points(pc);
+            off_by_sph[s] = off[r];
+            maxpts = std::max(maxpts, std::max(grid_points(p), grid_points(pc)));
+        }
+        const int64_t nc = off[nsph_];
+        std::vector<double> cc(3 * nc), cn(3 * nc), cw(nc);
+        for (int s = 0; s < nsph_; s++) {
+            const ShTables &t = sh_tables(level_order(l, order_[s]));
+            for (int q = 0; q < t.npts; q++) {
+                const int64_t g = off_by_sph[s] + q;
+                double nv[3];
+                qrot_host(&quat_[4 * s], &t.eta[3 * q], nv);
+                for (int c = 0; c < 3; c++) {
+                    cn[3 * g + c] = nv[c];
+                    cc[3 * g + c] = pos_[3 * s + c] + rad_[s] * nv[c];
+                }
+                cw[g] = rad_[s] * rad_[s] * t.w[q];
+            }
+        }
+        L->npts = nc;
+        L->max_pts = maxpts;
+        L->d_off_by_sph.upload(off_by_sph);
+        L->coords.upload(cc);
+        L->norms.upload(cn);
+        L->w.upload(cw);
+        std::vector<bool> used(kMaxOrder + 1, false);
+        for (int p : order_) used[p] = true;
+        for (int p = 0; p <= kMaxOrder; p++)
+            if (used[p]) {
+                const Resample &rs = resample_tables(p, level_order(l, p));
+                L->d_down[p].upload(rs.down);
+                L->d_up[p].upload(rs.up);
+                L->down.M[p] = L->d_down[p].p;
+                L->down.nin[p] = rs.n;
+                L->down.nout[p] = rs.nc;
+                L->up.M[p] = L->d_up[p].p;
+                L->up.nin[p] = rs.nc;
+                L->up.nout[p] = rs.n;
+            }
+        const bool last = (l == kNumLevels - 1);
+        build_far_lists(L->lists, nsusp_, susp_sph_off_, off, off, shell[l], !last, tpl_, 0, kWantItems);
+        std::vector<int> dummy;
+        L->chunks.upload(make_chunks(nsusp_, susp_sph_off_, off, &dummy));
+        L->sv.alloc(std::max<int64_t>(3 * nc, 1));
+        L->tv.alloc(std::max<int64_t>(3 * nc, 1));
+        L->src32.alloc(std::max<int64_t>(8 * nc, 8));
+        L->trg32.alloc(std::max<int64_t>(8 * nc, 8));
+        launch_pack_trg_f32(nc, L->coords.p, L->norms.p, L->trg32.p, stream_);
+        levels_.push_back(std::move(L));
+    }
+    levels_ready_ = true;
 }
 
 void Operator::harvest_events() {
@@ -530,8 +631,8 @@ void Operator::far_run(FarLists &L, int kind, int prec, const void *src, const d
 }
 
 // Far field of `in` into out (accumulating).  FP64: fine grid, direct sum over
-// all far pairs.  FP32: two-level (fine for mid-range pairs, coarse grids for
-// the rest) unless disabled, in which case the FP32 direct sum.
+// all far pairs.  FP32: multi-level (fine for mid-range pairs, coarser grids
+// with distance) unless disabled, in which case the FP32 direct sum.
 void Operator::far(int kind, int prec, double coef, const double *in, double *out, const int *active) {
     const bool timing = (flags_ & 1) != 0;
     EvPair ev{};
@@ -558,16 +659,18 @@ void Operator::far(int kind, int prec, double coef, const double *in, double *ou
             launch_pack_trg_f32(npts_, d_coords_.p, d_norms_.p, trg32_.p, stream_);
         }
         launch_pack_src_f32(npts_, d_coords_.p, d_w_.p, in, src32_.p, stream_);
-        if (two_level_) {
-            ensure_two_level();
-            const int64_t nc = pc_off_[nsph_];
-            launch_resample(geom_, down_tab_, d_pt_off_.p, d_pc_off_.p, in, sc_.p, 0, active, stream_,
-                            max_rs_points_);
-            launch_pack_src_f32(nc, d_ccoords_.p, d_cw_.p, sc_.p, src32c_.p, stream_);
-            launch_fill(3 * nc, 0.0, tc_.p, stream_);
+        if (multi_level_) {
+            ensure_levels();
+            for (auto &L : levels_) {
+                launch_resample(geom_, L->down, d_pt_off_.p, L->d_off_by_sph.p, in, L->sv.p, 0, active, stream_,
+                                L->max_pts);
+                launch_pack_src_f32(L->npts, L->coords.p, L->w.p, L->sv.p, L->src32.p, stream_);
+                launch_fill(3 * L->npts, 0.0, L->tv.p, stream_);
+            }
             if (timing) SPHSTK_HIP(hipEventRecord(ev.a, stream_));
-            far_run(lists_coarse_, kind, 1, src32c_.p, nullptr, nullptr, trg32c_.p, cchunks_.p, (int)cchunks_.n,
-                    coef, tc_.p, active);
+            for (auto &L : levels_)
+                far_run(L->lists, kind, 1, L->src32.p, nullptr, nullptr, L->trg32.p, L->chunks.p, (int)L->chunks.n,
+                        coef, L->tv.p, active);
             far_run(lists_mid_, kind, 1, src32_.p, nullptr, nullptr, trg32_.p, chunks_.p, (int)chunks_.n, coef, out,
                     active);
             if (timing) {
@@ -575,7 +678,9 @@ void Operator::far(int kind, int prec, double coef, const double *in, double *ou
                 pending_.push_back(ev);
                 timing_done_ = true;
             }
-            launch_resample(geom_, up_tab_, d_pc_off_.p, d_pt_off_.p, tc_.p, out, 1, active, stream_, max_rs_points_);
+            for (auto &L : levels_)
+                launch_resample(geom_, L->up, L->d_off_by_sph.p, d_pt_off_.p, L->tv.p, out, 1, active, stream_,
+                                L->max_pts);
         } else {
             if (timing) SPHSTK_HIP(hipEventRecord(ev.a, stream_));
             far_run(lists_fine_, kind, 1, src32_.p, nullptr, nullptr, trg32_.p, chunks_.p, (int)chunks_.n, coef, out,
@@ -604,14 +709,52 @@ void Operator::p2p(const double *in, double *out, double cId, double cSL, double
     if (doT) {
         far(0, prec, -3.0 / (4.0 * kPi) * cTrac, in, out, active);
         launch_expand(geom_, 1, coef_.p, rec_.p, active, stream_, max_order_);
-        launch_nearself(geom_, 1, rec_.p, in, out, 0.0, cTrac, cid, active, stream_);
+        launch_sheval(geom_, 1, rec_.p, items_.p, (int)items_.n, d_trg_slot_off_.p, d_trg_nsrc_.p, evscratch_.p, in,
+                      out, 0.0, cTrac, cid, active, stream_);
         cid = 0.0;
     }
     if (doSL) {
         far(1, prec, cSL / (8.0 * kPi), in, out, active);
         launch_expand(geom_, 0, coef_.p, rec_.p, active, stream_, max_order_);
-        launch_nearself(geom_, 0, rec_.p, in, out, cSL, 0.0, cid, active, stream_);
+        launch_sheval(geom_, 0, rec_.p, items_.p, (int)items_.n, d_trg_slot_off_.p, d_trg_nsrc_.p, evscratch_.p, in,
+                      out, cSL, 0.0, cid, active, stream_);
     }
+    SPHSTK_HIP(hipGetLastError());
+}
+
+// Rigid-body averages of u = S[rho] (V = mean u / mu, Omega = 3/(8 pi R^4 mu) sum w R n x u):
+// self and near terms by quadrature of the SH evaluations on the grid (as the
+// reference), far pairs through the sphere-centre sums (Faxen's laws).
+void Operator::rigid_velocity(const double *rho, double inv_mu, double *vel) {
+    if (nsph_ == 0) return;
+    if (!centre_ready_) {
+        std::vector<int64_t> toff(nsph_ + 1);
+        for (int i = 0; i <= nsph_; i++) toff[i] = i;
+        std::vector<std::vector<int>> excl(nsph_);
+        for (int t = 0; t < nsph_; t++) {
+            excl[t].push_back(t);
+            for (int64_t k = near_off_[t]; k < near_off_[t + 1]; k++) excl[t].push_back(near_src_[k]);
+            std::sort(excl[t].begin(), excl[t].end());
+        }
+        int64_t nb = 0;
+        for (int b = 0; b < nsusp_; b++) nb += (susp_sph_off_[b + 1] - susp_sph_off_[b] + 63) / 64;
+        const int split = (int)std::max<int64_t>(1, std::min<int64_t>(64, (kWantItems + nb - 1) / std::max<int64_t>(nb, 1)));
+        build_far_lists(lists_centre_, nsusp_, susp_sph_off_, toff, pt_off_, excl, false, 1, split, kWantItems);
+        centre_part_.alloc(size_t(lists_centre_.split) * 9 * nsph_);
+        vel_ns_.alloc(6 * (size_t)nsph_);
+        u_ns_.alloc(std::max<int64_t>(3 * npts_, 1));
+        centre_ready_ = true;
+    }
+    launch_analysis(geom_, rho, coef_.p, nullptr, stream_, max_order_);
+    launch_expand(geom_, 0, coef_.p, rec_.p, nullptr, stream_, max_order_);
+    launch_fill(3 * npts_, 0.0, u_ns_.p, stream_);
+    launch_sheval(geom_, 0, rec_.p, items_.p, (int)items_.n, d_trg_slot_off_.p, d_trg_nsrc_.p, evscratch_.p, rho,
+                  u_ns_.p, 1.0, 0.0, 0.0, nullptr, stream_);
+    launch_rigid_avg(geom_, u_ns_.p, inv_mu, vel_ns_.p, stream_);
+    launch_pack_src_f64(npts_, d_coords_.p, d_w_.p, rho, src64_.p, stream_);
+    launch_far_centre(src64_.p, d_pos_.p, lists_centre_.ngroups, lists_centre_.order.p, lists_centre_.groups.p,
+                      lists_centre_.iv.p, centre_part_.p, nsph_, stream_);
+    launch_centre_finish(nsph_, lists_centre_.split, centre_part_.p, d_rad_.p, vel_ns_.p, inv_mu, vel, stream_);
     SPHSTK_HIP(hipGetLastError());
 }
 
@@ -636,7 +779,7 @@ void Operator::project(double *inout) {
     if (nsph_ == 0) return;
     launch_analysis(geom_, inout, coef_.p, nullptr, stream_, max_order_);
     launch_expand(geom_, 1, coef_.p, rec_.p, nullptr, stream_, max_order_);
-    launch_synthesis(geom_, rec_.p, inout, stream_);
+    launch_synthesis(geom_, rec_.p, self_items_.p, (int)self_items_.n, inout, stream_);
     SPHSTK_HIP(hipGetLastError());
 }
 
@@ -665,8 +808,10 @@ void Operator::kernel_stats(int kind, int prec, double *ms, int64_t *launches, d
     if (ms) *ms = stats_[kind][prec].ms;
     if (launches) *launches = stats_[kind][prec].launches;
     if (pairs) {
-        if (prec == 1 && two_level_ && two_level_ready_)
-            *pairs = lists_mid_.pairs + lists_coarse_.pairs;
+        if (prec == 1 && multi_level_ && levels_ready_) {
+            *pairs = lists_mid_.pairs;
+            for (auto &L : levels_) *pairs += L->lists.pairs;
+        }
         else
             *pairs = lists_fine_.pairs;
     }
@@ -986,10 +1131,14 @@ void Mobility::solve(const double *force, double *vel, std::vector<SegInfo> &inf
     x_zero_ = false;
     // rho = rho0 + x; u = S[rho]; rigid-body averages / mu   (:253-285)
     launch_axpby(n, 1.0, x_.p, 1.0, rho_.p, st);
-    launch_fill(n, 0.0, b_.p, st);
-    A.p2p(rho_.p, b_.p, 0.0, 1.0, 0.0, 0, nullptr);
-    app64++;
-    launch_rigid_avg(A.geom(), b_.p, 1.0 / mu_, vel_.p, st);
+    if (prec_ == 0) {
+        launch_fill(n, 0.0, b_.p, st);
+        A.p2p(rho_.p, b_.p, 0.0, 1.0, 0.0, 0, nullptr);
+        app64++;
+        launch_rigid_avg(A.geom(), b_.p, 1.0 / mu_, vel_.p, st);
+    } else {
+        A.rigid_velocity(rho_.p, 1.0 / mu_, vel_.p);
+    }
     SPHSTK_HIP(hipMemcpyAsync(vel, vel_.p, sizeof(double) * 6 * N, hipMemcpyDeviceToHost, st));
     SPHSTK_HIP(hipStreamSynchronize(st));
 }

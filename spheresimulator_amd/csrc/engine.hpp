@@ -52,6 +52,20 @@ void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sp
                      const std::vector<std::vector<int>> &lists, bool include, int tpl, int split,
                      int64_t want_items);
 
+// One level of the multi-level FP32 far field: per-sphere grids of a lower
+// order, their geometry, resampling tables and work lists.
+struct FarLevel {
+    int64_t npts = 0;
+    int max_pts = 0;
+    DevBuf<int64_t> d_off_by_sph; // first level point of each sphere
+    DevBuf<double> coords, norms, w, sv, tv;
+    DevBuf<float> src32, trg32;
+    DevBuf<Chunk> chunks;
+    DevBuf<double> d_down[kMaxOrder + 1], d_up[kMaxOrder + 1];
+    ResampleTabs down, up;
+    FarLists lists;
+};
+
 struct KernelStats {
     double ms = 0.0;
     int64_t launches = 0;
@@ -98,6 +112,8 @@ class Operator {
     // Y = beta Y + alpha A X
     void apply(const double *X, double *Y, double alpha, double beta, int prec);
     void project(double *inout);
+    // rigid-body velocities (6 per sphere) of the single layer of rho, far pairs by Faxen's laws
+    void rigid_velocity(const double *rho, double inv_mu, double *vel);
 
     // host accessors
     void grid(double *weights, double *coords, double *norms, int64_t *index, int64_t *length) const;
@@ -119,7 +135,7 @@ class Operator {
     void far(int kind, int prec, double coef, const double *in, double *out, const int *active);
     void far_run(FarLists &L, int kind, int prec, const void *src, const double *tcoords, const double *tnorms,
                  const float *trg32, const Chunk *chunks, int nchunks, double coef, double *out, const int *active);
-    void ensure_two_level();
+    void ensure_levels();
 
     int nsph_, nsusp_;
     int64_t npts_;
@@ -143,19 +159,22 @@ class Operator {
     DevBuf<double> d_anal_[kMaxOrder + 1];
     DevGeom geom_;
 
-    // far-field work lists: fine grid (FP64 and direct FP32), and the two-level
-    // FP32 pair (fine mid-range pairs, coarse-grid pairs)
-    FarLists lists_fine_, lists_mid_, lists_coarse_;
-    bool two_level_ = true, two_level_ready_ = false, timing_done_ = false;
-    double alpha_ = 4.0;
-    int max_rs_points_ = 0;
-    std::vector<int64_t> pc_off_;
-    DevBuf<int64_t> d_pc_off_;
-    DevBuf<double> d_ccoords_, d_cnorms_, d_cw_, sc_, tc_;
-    DevBuf<float> src32c_, trg32c_;
-    DevBuf<Chunk> cchunks_;
-    DevBuf<double> d_down_[kMaxOrder + 1], d_up_[kMaxOrder + 1];
-    ResampleTabs down_tab_, up_tab_;
+    // far-field work lists: fine grid (FP64 and direct FP32); the multi-level
+    // FP32 far field (fine mid-range pairs + coarser levels)
+    FarLists lists_fine_, lists_mid_;
+    std::vector<std::unique_ptr<FarLevel>> levels_;
+    bool multi_level_ = true, levels_ready_ = false, timing_done_ = false;
+
+    // sphere-centre far field for rigid_velocity
+    FarLists lists_centre_;
+    bool centre_ready_ = false;
+    DevBuf<double> centre_part_, vel_ns_, u_ns_;
+
+    // self/near evaluation work items and their scratch
+    DevBuf<EvalItem> items_, self_items_;
+    DevBuf<int64_t> d_trg_slot_off_;
+    DevBuf<int> d_trg_nsrc_;
+    DevBuf<double> evscratch_;
 
     // per-apply scratch
     DevBuf<double> src64_, coef_, rec_, tmp_;

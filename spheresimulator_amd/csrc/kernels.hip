@@ -701,88 +701,103 @@ __device__ __forceinline__ void traction_from(const double *A, const double x[3]
                2.0 * h3 * v[c] + x[c] * vg3 + xv * g3[c] + xH4v[c] + H4vx[c];
 }
 
-// MODE 0: single layer, 1: traction, 2: synthesis of the projected density (self only)
+// One work item = one 64-lane wave: targets [tstart, tstart+tcount) of sphere T
+// against the expansion of sphere S (S == T: self).  MODE 0: single layer,
+// 1: traction, 2: synthesis of the projected density (self items only).
+// MODE 0/1 write the item's contribution (lab frame, coefficients applied) to
+// scratch[slot + 3*lane]; MODE 2 writes out directly.
 template <int MODE>
-__global__ __launch_bounds__(128) void k_nearself(DevGeom g, const double *__restrict__ rec,
-                                                  const double *__restrict__ in, double *__restrict__ out,
-                                                  double cSL, double cTrac, double cId,
-                                                  const int *__restrict__ active) {
+__global__ __launch_bounds__(64) void k_sheval(DevGeom g, const double *__restrict__ rec,
+                                               const EvalItem *__restrict__ items, double *__restrict__ scratch,
+                                               double *__restrict__ out, double cSL, double cTrac,
+                                               const int *__restrict__ active) {
+    const EvalItem it = items[blockIdx.x];
+    const int T = it.trg, S = it.src;
+    if (active && !active[g.sph_susp[T]]) return;
+    const int lane = threadIdx.x;
+    if (lane >= it.tcount) return;
+    const int64_t gp = g.pt_off[T] + it.tstart + lane;
+    const double X[3] = {g.coords[3 * gp], g.coords[3 * gp + 1], g.coords[3 * gp + 2]};
+    const double *q = g.quat + 4 * S;
+    const double R = g.rad[S];
+    const double iR = 1.0 / R;
+    const double d[3] = {(X[0] - g.pos[3 * S]) * iR, (X[1] - g.pos[3 * S + 1]) * iR, (X[2] - g.pos[3 * S + 2]) * iR};
+    double xi[3], res[3], lab[3];
+    qrot(q, d, xi, true);
+    const int P = g.order[S] + 2;
+    const double *recS = rec + g.rec_off[S];
+    if (MODE == 0) {
+        double A[kRecSL];
+        eval_record<kRecSL, 0, kRecSL>(recS, P, xi[0], xi[1], xi[2], A);
+        const double r2 = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+        double xg4[3];
+        cross3(xi, A + 7, xg4);
+        for (int c = 0; c < 3; c++) res[c] = A[c] + r2 * A[3 + c] + xi[c] * A[6] + xg4[c];
+        qrot(q, res, lab, false);
+        for (int c = 0; c < 3; c++) scratch[it.slot + 3 * lane + c] = cSL * R * lab[c];
+    } else if (MODE == 1) {
+        const double Nl[3] = {g.norms[3 * gp], g.norms[3 * gp + 1], g.norms[3 * gp + 2]};
+        double vb[3];
+        qrot(q, Nl, vb, true);
+        if (S == T) {
+            double A[kRecTrac];
+            eval_record<kRecTrac, 0, kRecTrac>(recS, P, xi[0], xi[1], xi[2], A);
+            traction_from(A, xi, vb, res);
+            double xq[3];
+            cross3(xi, A + 30, xq);
+            for (int c = 0; c < 3; c++) res[c] += 0.5 * (A[26] * xi[c] + A[27 + c] + xq[c]);
+        } else {
+            double A[kRecTracNear];
+            eval_record<kRecTrac, 0, kRecTracNear>(recS, P, xi[0], xi[1], xi[2], A);
+            traction_from(A, xi, vb, res);
+        }
+        qrot(q, res, lab, false);
+        for (int c = 0; c < 3; c++) scratch[it.slot + 3 * lane + c] = cTrac * lab[c];
+    } else {
+        double A[7];
+        eval_record<kRecTrac, 26, 7>(recS, P, xi[0], xi[1], xi[2], A);
+        double xq[3];
+        cross3(xi, A + 4, xq);
+        for (int c = 0; c < 3; c++) res[c] = A[0] * xi[c] + A[1 + c] + xq[c];
+        qrot(q, res, lab, false);
+        for (int c = 0; c < 3; c++) out[3 * gp + c] = lab[c];
+    }
+}
+
+// out[T's points] += sum over T's items (self first, then near pairs in list
+// order) of their scratch contributions + cId in
+__global__ __launch_bounds__(256) void k_sheval_gather(DevGeom g, const int64_t *__restrict__ trg_slot_off,
+                                                       const int *__restrict__ trg_nsrc, const double *__restrict__ scratch,
+                                                       const double *__restrict__ in, double *__restrict__ out, double cId,
+                                                       const int *__restrict__ active) {
     const int T = blockIdx.x;
     if (active && !active[g.sph_susp[T]]) return;
     const int64_t off = g.pt_off[T];
     const int nT = (int)(g.pt_off[T + 1] - off);
-    const int nnear = (MODE == 2) ? 0 : (int)(g.near_off[T + 1] - g.near_off[T]);
-    for (int j = threadIdx.x; j < nT; j += blockDim.x) {
-        const int64_t gp = off + j;
-        const double X[3] = {g.coords[3 * gp], g.coords[3 * gp + 1], g.coords[3 * gp + 2]};
-        const double Nl[3] = {g.norms[3 * gp], g.norms[3 * gp + 1], g.norms[3 * gp + 2]};
-        double o[3] = {0.0, 0.0, 0.0};
-        for (int si = 0; si <= nnear; si++) {
-            const int S = (si == 0) ? T : g.near_src[g.near_off[T] + si - 1];
-            const double *q = g.quat + 4 * S;
-            const double R = g.rad[S];
-            const double iR = 1.0 / R;
-            const double d[3] = {(X[0] - g.pos[3 * S]) * iR, (X[1] - g.pos[3 * S + 1]) * iR,
-                                 (X[2] - g.pos[3 * S + 2]) * iR};
-            double xi[3], vb[3], res[3], lab[3];
-            qrot(q, d, xi, true);
-            const int P = g.order[S] + 2;
-            const double *recS = rec + g.rec_off[S];
-            if (MODE == 0) {
-                double A[kRecSL];
-                eval_record<kRecSL, 0, kRecSL>(recS, P, xi[0], xi[1], xi[2], A);
-                const double r2 = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
-                double xg4[3];
-                cross3(xi, A + 7, xg4);
-                for (int c = 0; c < 3; c++) res[c] = A[c] + r2 * A[3 + c] + xi[c] * A[6] + xg4[c];
-                qrot(q, res, lab, false);
-                for (int c = 0; c < 3; c++) o[c] += cSL * R * lab[c];
-            } else if (MODE == 1) {
-                qrot(q, Nl, vb, true);
-                if (si == 0) {
-                    double A[kRecTrac];
-                    eval_record<kRecTrac, 0, kRecTrac>(recS, P, xi[0], xi[1], xi[2], A);
-                    traction_from(A, xi, vb, res);
-                    double xq[3];
-                    cross3(xi, A + 30, xq);
-                    for (int c = 0; c < 3; c++) res[c] += 0.5 * (A[26] * xi[c] + A[27 + c] + xq[c]);
-                } else {
-                    double A[kRecTracNear];
-                    eval_record<kRecTrac, 0, kRecTracNear>(recS, P, xi[0], xi[1], xi[2], A);
-                    traction_from(A, xi, vb, res);
-                }
-                qrot(q, res, lab, false);
-                for (int c = 0; c < 3; c++) o[c] += cTrac * lab[c];
-            } else {
-                double A[7];
-                eval_record<kRecTrac, 26, 7>(recS, P, xi[0], xi[1], xi[2], A);
-                double xq[3];
-                cross3(xi, A + 4, xq);
-                for (int c = 0; c < 3; c++) res[c] = A[0] * xi[c] + A[1 + c] + xq[c];
-                qrot(q, res, lab, false);
-                for (int c = 0; c < 3; c++) o[c] = lab[c];
-            }
-        }
-        if (MODE == 2) {
-            for (int c = 0; c < 3; c++) out[3 * gp + c] = o[c];
-        } else {
-            for (int c = 0; c < 3; c++) out[3 * gp + c] += o[c] + cId * in[3 * gp + c];
-        }
+    const int64_t base = trg_slot_off[T];
+    const int ns = trg_nsrc[T];
+    for (int i = threadIdx.x; i < 3 * nT; i += blockDim.x) {
+        double s = 0.0;
+        for (int k = 0; k < ns; k++) s += scratch[base + (int64_t)k * 3 * nT + i];
+        out[3 * off + i] += s + cId * in[3 * off + i];
     }
 }
 
-void launch_nearself(const DevGeom &g, int mode, const double *rec, const double *in, double *out, double cSL,
-                     double cTrac, double cId, const int *active, hipStream_t s) {
-    if (!g.nsph) return;
+void launch_sheval(const DevGeom &g, int mode, const double *rec, const EvalItem *items, int nitems,
+                   const int64_t *trg_slot_off, const int *trg_nsrc, double *scratch, const double *in, double *out,
+                   double cSL, double cTrac, double cId, const int *active, hipStream_t s) {
+    if (!g.nsph || nitems <= 0) return;
     if (mode == 0)
-        k_nearself<0><<<dim3(g.nsph), dim3(128), 0, s>>>(g, rec, in, out, cSL, cTrac, cId, active);
+        k_sheval<0><<<dim3(nitems), dim3(64), 0, s>>>(g, rec, items, scratch, out, cSL, cTrac, active);
     else
-        k_nearself<1><<<dim3(g.nsph), dim3(128), 0, s>>>(g, rec, in, out, cSL, cTrac, cId, active);
+        k_sheval<1><<<dim3(nitems), dim3(64), 0, s>>>(g, rec, items, scratch, out, cSL, cTrac, active);
+    k_sheval_gather<<<dim3(g.nsph), dim3(256), 0, s>>>(g, trg_slot_off, trg_nsrc, scratch, in, out, cId, active);
 }
 
-void launch_synthesis(const DevGeom &g, const double *rec, double *out, hipStream_t s) {
-    if (!g.nsph) return;
-    k_nearself<2><<<dim3(g.nsph), dim3(128), 0, s>>>(g, rec, nullptr, out, 0.0, 0.0, 0.0, nullptr);
+void launch_synthesis(const DevGeom &g, const double *rec, const EvalItem *self_items, int nitems, double *out,
+                      hipStream_t s) {
+    if (!g.nsph || nitems <= 0) return;
+    k_sheval<2><<<dim3(nitems), dim3(64), 0, s>>>(g, rec, self_items, nullptr, out, 0.0, 0.0, nullptr);
 }
 
 __global__ void k_identity(DevGeom g, const double *__restrict__ in, double *__restrict__ out, double cId,
@@ -953,6 +968,82 @@ void launch_axpby(int64_t n, double a, const double *x, double b, double *y, hip
 }
 void launch_fill(int64_t n, double v, double *y, hipStream_t s) {
     if (n > 0) k_fill<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(n, v, y);
+}
+
+// ---------------------------------------------------------------------------
+// Sphere-centre far field for the rigid-body averages of a single-layer flow
+// (Faxen's laws, exact for flows whose singularities lie outside the sphere):
+//   mean of u over the sphere = u(c) + R^2/6 lap u(c),   rotation = curl u(c) / 2.
+// For a Stokeslet of strength f at y (r = c - y), times 1/(8 pi):
+//   u = f/r + r (r.f)/r^3,  lap u = 2 f/r^3 - 6 r (r.f)/r^5,  curl u / 2 = (f x r)/r^3.
+// Targets are sphere centres; out[9*T + (0..2: u, 3..5: lap u, 6..8: curl/2)] or,
+// with slots, part[slot*9*ntrg + ...].
+__global__ __launch_bounds__(64) void k_far_centre_f64(const double *__restrict__ src,
+                                                       const double *__restrict__ centres,
+                                                       const int *__restrict__ gorder,
+                                                       const FarGroup *__restrict__ groups,
+                                                       const FarInterval *__restrict__ iv, double *__restrict__ part,
+                                                       int64_t ntrg) {
+    const FarGroup G = groups[gorder[blockIdx.x]];
+    const int lane = threadIdx.x;
+    const int64_t tid = G.tstart + (lane < G.tcount ? lane : 0);
+    const double x = centres[3 * tid], y = centres[3 * tid + 1], z = centres[3 * tid + 2];
+    double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t k = G.iv_begin; k < G.iv_end; k++) {
+        const bool incl = ((iv[k].mask[0] >> lane) & 1ull) != 0;
+        for (int64_t s = iv[k].begin; s != iv[k].end; s++) {
+            const double *sp = src + 8 * s;
+            const double dx = x - sp[0], dy = y - sp[1], dz = z - sp[2];
+            const double fx = sp[3], fy = sp[4], fz = sp[5];
+            const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
+            double ri = rsqrt_f64(r2);
+            ri = incl ? ri : 0.0;
+            const double ri2 = ri * ri, ri3 = ri2 * ri, ri5 = ri3 * ri2;
+            const double df = __builtin_fma(dx, fx, __builtin_fma(dy, fy, dz * fz));
+            const double q3 = df * ri3, q5 = -6.0 * df * ri5, t3 = 2.0 * ri3;
+            a[0] = __builtin_fma(fx, ri, __builtin_fma(q3, dx, a[0]));
+            a[1] = __builtin_fma(fy, ri, __builtin_fma(q3, dy, a[1]));
+            a[2] = __builtin_fma(fz, ri, __builtin_fma(q3, dz, a[2]));
+            a[3] = __builtin_fma(fx, t3, __builtin_fma(q5, dx, a[3]));
+            a[4] = __builtin_fma(fy, t3, __builtin_fma(q5, dy, a[4]));
+            a[5] = __builtin_fma(fz, t3, __builtin_fma(q5, dz, a[5]));
+            a[6] = __builtin_fma(fy * dz - fz * dy, ri3, a[6]);
+            a[7] = __builtin_fma(fz * dx - fx * dz, ri3, a[7]);
+            a[8] = __builtin_fma(fx * dy - fy * dx, ri3, a[8]);
+        }
+    }
+    if (lane < G.tcount) {
+        double *o = part + G.slot * 9 * ntrg + 9 * (G.tstart + lane);
+        for (int c = 0; c < 9; c++) o[c] = a[c];
+    }
+}
+
+// vel[6T..] = (near/self rigid averages) + far part from the centre sums, / mu
+__global__ void k_centre_finish(int nsph, int nslot, const double *__restrict__ part, const double *__restrict__ rad,
+                                const double *__restrict__ vel_ns, double inv_mu, double *__restrict__ vel) {
+    const int T = blockIdx.x * blockDim.x + threadIdx.x;
+    if (T >= nsph) return;
+    double a[9];
+    for (int c = 0; c < 9; c++) {
+        double s = 0.0;
+        for (int k = 0; k < nslot; k++) s += part[(int64_t)k * 9 * nsph + 9 * T + c];
+        a[c] = s;
+    }
+    const double R = rad[T], f = 1.0 / (8.0 * kPi);
+    for (int c = 0; c < 3; c++) {
+        vel[6 * T + c] = vel_ns[6 * T + c] + f * (a[c] + R * R / 6.0 * a[3 + c]) * inv_mu;
+        vel[6 * T + 3 + c] = vel_ns[6 * T + 3 + c] + f * a[6 + c] * inv_mu;
+    }
+}
+
+void launch_far_centre(const double *src, const double *centres, int ngroups, const int *go, const FarGroup *gr,
+                       const FarInterval *iv, double *part, int64_t ntrg, hipStream_t s) {
+    if (ngroups > 0) k_far_centre_f64<<<dim3(ngroups), dim3(64), 0, s>>>(src, centres, go, gr, iv, part, ntrg);
+}
+void launch_centre_finish(int nsph, int nslot, const double *part, const double *rad, const double *vel_ns,
+                          double inv_mu, double *vel, hipStream_t s) {
+    if (nsph > 0)
+        k_centre_finish<<<dim3((nsph + 127) / 128), dim3(128), 0, s>>>(nsph, nslot, part, rad, vel_ns, inv_mu, vel);
 }
 
 // ---------------------------------------------------------------------------

@@ -64,14 +64,25 @@ void launch_analysis(const DevGeom &g, const double *in, double *coef, const int
 // mode 0: single-layer record (kRecSL arrays), 1: traction record (kRecTrac arrays)
 void launch_expand(const DevGeom &g, int mode, const double *coef, double *rec, const int *active, hipStream_t s,
                    int max_order);
-// out += cSL*S_self/near + cTrac*K_self/near + cId*in   (mode as above)
-void launch_nearself(const DevGeom &g, int mode, const double *rec, const double *in, double *out, double cSL,
-                     double cTrac, double cId, const int *active, hipStream_t s);
+// One self/near evaluation work item: targets [tstart, tstart+tcount) (<= 64) of
+// sphere trg against the expansion of sphere src; its contribution goes to
+// scratch[slot ...] (3 doubles per target).
+struct EvalItem {
+    int32_t trg, src;
+    int32_t tstart, tcount;
+    int64_t slot;
+};
+
+// out += cSL*S_self/near + cTrac*K_self/near + cId*in   (mode 0: single layer, 1: traction)
+void launch_sheval(const DevGeom &g, int mode, const double *rec, const EvalItem *items, int nitems,
+                   const int64_t *trg_slot_off, const int *trg_nsrc, double *scratch, const double *in, double *out,
+                   double cSL, double cTrac, double cId, const int *active, hipStream_t s);
 void launch_identity(const DevGeom &g, const double *in, double *out, double cId, const int *active,
                      hipStream_t s);
 void launch_lop(const DevGeom &g, const double *in, double *out, double cLOP, const int *active, hipStream_t s);
-// out = P in (grid values of the band-limited projection) from a traction record
-void launch_synthesis(const DevGeom &g, const double *rec, double *out, hipStream_t s);
+// out = P in (grid values of the band-limited projection) from a traction record (self items only)
+void launch_synthesis(const DevGeom &g, const double *rec, const EvalItem *self_items, int nitems, double *out,
+                      hipStream_t s);
 
 // segmented BLAS-1 over chunks (one segment per suspension)
 void launch_seg_dot(const double *a, const double *b, const Chunk *chunks, int nchunks, double *partial,
@@ -95,6 +106,11 @@ void launch_slot_reduce(double *out, const double *part, int nslot, int64_t stri
 void launch_axpby(int64_t n, double a, const double *x, double b, double *y, hipStream_t s);
 void launch_fill(int64_t n, double v, double *y, hipStream_t s);
 
+// sphere-centre far field (Faxen) of packed FP64 sources, 9 sums per centre and slot
+void launch_far_centre(const double *src, const double *centres, int ngroups, const int *go, const FarGroup *gr,
+                       const FarInterval *iv, double *part, int64_t ntrg, hipStream_t s);
+void launch_centre_finish(int nsph, int nslot, const double *part, const double *rad, const double *vel_ns,
+                          double inv_mu, double *vel, hipStream_t s);
 void launch_rho0(const DevGeom &g, const double *force, double *rho, hipStream_t s);
 void launch_rigid_avg(const DevGeom &g, const double *u, double inv_mu, double *vel, hipStream_t s);
 

@@ -178,15 +178,22 @@ static std::vector<int> coef_rows_upto(int p, int pc) {
     return r;
 }
 
-static std::unique_ptr<Resample> build_resample(int p) {
+static std::unique_ptr<Resample> build_resample(int p, int pc) {
     auto rs = std::make_unique<Resample>();
     rs->p = p;
-    rs->pc = coarse_order(p);
+    rs->pc = pc;
     const ShTables &F = sh_tables(p), &C = sh_tables(rs->pc);
     rs->n = F.npts;
     rs->nc = C.npts;
     const int QF = F.Q, QC = C.Q;
     const int rowsF = 6 * QF, rowsC = 6 * QC;
+    if (rs->pc == p) { // same grid: identity in both directions
+        const int nn = 3 * F.npts;
+        rs->down.assign(size_t(nn) * nn, 0.0);
+        for (int i = 0; i < nn; i++) rs->down[size_t(i) * nn + i] = 1.0;
+        rs->up = rs->down;
+        return rs;
+    }
     const std::vector<int> rf = coef_rows_upto(p, rs->pc), rc = coef_rows_upto(rs->pc, rs->pc);
     // down = synth_C[:, rc] * anal_F[rf, :]
     const int nF = 3 * F.npts, nC = 3 * C.npts;
@@ -212,15 +219,15 @@ static std::unique_ptr<Resample> build_resample(int p) {
     return rs;
 }
 
-const Resample &resample_tables(int p) {
+const Resample &resample_tables(int p, int pc) {
     static std::mutex mu;
-    static std::map<int, std::unique_ptr<Resample>> cache;
+    static std::map<std::pair<int, int>, std::unique_ptr<Resample>> cache;
     std::lock_guard<std::mutex> lock(mu);
-    auto it = cache.find(p);
+    auto it = cache.find({p, pc});
     if (it != cache.end()) return *it->second;
-    auto t = build_resample(p);
+    auto t = build_resample(p, pc);
     const Resample &ref = *t;
-    cache[p] = std::move(t);
+    cache[{p, pc}] = std::move(t);
     return ref;
 }
 
