@@ -110,10 +110,19 @@ int sphstk_operator_apply_device(sphstk_operator *op, const double *dX, double *
 
 /* The FP32 inner operator of the mixed-precision solve, Y := beta Y + alpha A32 X
  * (one column).  No counterpart in the reference: A32 has the far field in FP32
- * and, unless SPHSTK_FLAG_FP32_DIRECT was given at creation, two-level (sphere
- * pairs farther apart than 4 (R_s + R_t) interact through degree-p/2 coarse
- * grids); self and near terms stay FP64. */
+ * and, unless SPHSTK_FLAG_FP32_DIRECT was given at creation, multi-level (sphere
+ * pairs farther apart than 4 (R_s + R_t) interact through coarser grids of order
+ * p/2+1, max(5, p/3+1) and 4 from 4, 8 and 14 (R_s + R_t) on; orders below 10
+ * keep their own grid at the first level); self and near terms stay FP64. */
 int sphstk_operator_apply_inner(sphstk_operator *op, const double *X, double *Y, double alpha, double beta);
+
+/* The FP64 outer operator of the mixed-precision solve (its right-hand side and
+ * refinement residuals), Y := beta Y + alpha A64ml X (one column).  No
+ * counterpart in the reference (whose far field is an FMM to a set tolerance):
+ * FP64 pair sums with sphere pairs farther apart than 8 (R_s + R_t)
+ * interacting through coarser grids of order min(p, 9), min(p, 7) and
+ * min(p, 6) from 8, 16 and 32 (R_s + R_t) on; self and near terms as A64. */
+int sphstk_operator_apply_outer(sphstk_operator *op, const double *X, double *Y, double alpha, double beta);
 
 /* applyP2POP: out += cId in + cSL S[in] + cTrac K'[in]  (cpp:262-485; hpp:204-206). */
 int sphstk_operator_apply_p2p(sphstk_operator *op, const double *in, double *out, double cId, double cSL,
@@ -139,7 +148,8 @@ int sphstk_operator_get_near_pairs(const sphstk_operator *op, int32_t *trg, int3
 
 /* Far-field kernel timing collected with SPHSTK_FLAG_TIMING: total milliseconds,
  * launches, and the algorithmic pair count per launch, for the traction (kind 0)
- * or single-layer (kind 1) kernel in FP64 (prec 0) or FP32 (prec 1). */
+ * or single-layer (kind 1) pair sums of the FP64 operator (prec 0), the FP32
+ * inner operator (prec 1) or the FP64 multi-level outer operator (prec 2). */
 int sphstk_operator_kernel_stats(const sphstk_operator *op, int kind, int prec, double *total_ms, int64_t *launches,
                                  double *pairs_per_launch);
 int sphstk_operator_reset_stats(sphstk_operator *op);

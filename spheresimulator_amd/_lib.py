@@ -19,6 +19,7 @@ EXPORTED = [
     "sphstk_last_error", "sphstk_version", "sphstk_device_count",
     "sphstk_operator_create", "sphstk_operator_create_batched", "sphstk_operator_destroy",
     "sphstk_operator_sizes", "sphstk_operator_apply", "sphstk_operator_apply_device", "sphstk_operator_apply_inner",
+    "sphstk_operator_apply_outer",
     "sphstk_operator_apply_p2p", "sphstk_operator_apply_p2p_device", "sphstk_operator_apply_lop",
     "sphstk_operator_project_nullspace", "sphstk_operator_get_grid", "sphstk_operator_get_near_pairs",
     "sphstk_operator_kernel_stats", "sphstk_operator_reset_stats",
@@ -70,6 +71,7 @@ def load():
     L.sphstk_operator_apply.argtypes = [_vp, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.sphstk_operator_apply_device.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.sphstk_operator_apply_inner.argtypes = [_vp, _dp, _dp, ctypes.c_double, ctypes.c_double]
+    L.sphstk_operator_apply_outer.argtypes = [_vp, _dp, _dp, ctypes.c_double, ctypes.c_double]
     L.sphstk_operator_apply_p2p.argtypes = [_vp, _dp, _dp] + [ctypes.c_double] * 3
     L.sphstk_operator_apply_p2p_device.argtypes = [_vp, _vp, _vp] + [ctypes.c_double] * 3
     L.sphstk_operator_apply_lop.argtypes = [_vp, _dp, _dp, ctypes.c_double]

@@ -96,6 +96,15 @@ class SphereSTKSHOperator:
         check(load().sphstk_operator_apply_inner(self._h, _d(X), _d(Y), alpha, beta))
         return Y
 
+    def apply_outer(self, X, Y=None, alpha=1.0, beta=0.0):
+        """The FP64 multi-level outer operator of the mixed-precision solve (one column)."""
+        X = _f64(X, self.n_dof, "X")
+        Y = np.zeros(self.n_dof) if Y is None else Y
+        if Y.size != self.n_dof or Y.dtype != np.float64 or not Y.flags.c_contiguous:
+            raise ValueError("Y must be a contiguous float64 array of length n_dof")
+        check(load().sphstk_operator_apply_outer(self._h, _d(X), _d(Y), alpha, beta))
+        return Y
+
     def applyP2POP(self, inp, out, cId, cSL, cTrac):
         """out += cId in + cSL S[in] + cTrac K'[in]  (accumulates, as the reference)."""
         inp = _f64(inp, self.n_dof, "in")

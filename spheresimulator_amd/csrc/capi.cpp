@@ -178,7 +178,7 @@ int sphstk_operator_apply_device(sphstk_operator *op, const double *dX, double *
     });
 }
 
-int sphstk_operator_apply_inner(sphstk_operator *op, const double *X, double *Y, double alpha, double beta) {
+static int apply_prec(sphstk_operator *op, const double *X, double *Y, double alpha, double beta, int prec) {
     return guard([&] {
         SPHSTK_REQUIRE(op && op->op, "invalid operator handle");
         SPHSTK_REQUIRE(X && Y, "X and Y must not be NULL");
@@ -189,10 +189,18 @@ int sphstk_operator_apply_inner(sphstk_operator *op, const double *X, double *Y,
         dy.alloc(std::max<int64_t>(n, 1));
         SPHSTK_HIP(hipMemcpyAsync(dx.p, X, sizeof(double) * n, hipMemcpyHostToDevice, A.stream()));
         if (beta != 0.0) SPHSTK_HIP(hipMemcpyAsync(dy.p, Y, sizeof(double) * n, hipMemcpyHostToDevice, A.stream()));
-        A.apply(dx.p, dy.p, alpha, beta, 1);
+        A.apply(dx.p, dy.p, alpha, beta, prec);
         SPHSTK_HIP(hipMemcpyAsync(Y, dy.p, sizeof(double) * n, hipMemcpyDeviceToHost, A.stream()));
         SPHSTK_HIP(hipStreamSynchronize(A.stream()));
     });
+}
+
+int sphstk_operator_apply_inner(sphstk_operator *op, const double *X, double *Y, double alpha, double beta) {
+    return apply_prec(op, X, Y, alpha, beta, 1);
+}
+
+int sphstk_operator_apply_outer(sphstk_operator *op, const double *X, double *Y, double alpha, double beta) {
+    return apply_prec(op, X, Y, alpha, beta, 2);
 }
 
 int sphstk_operator_apply_p2p(sphstk_operator *op, const double *in, double *out, double cId, double cSL,

@@ -238,8 +238,32 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     if (const char *v = std::getenv("SPHSTK_FAR_TPL")) tpl = std::atoi(v);
     if (const char *v = std::getenv("SPHSTK_FAR_SPLIT")) split = std::atoi(v);
     tpl_ = tpl;
+    // FP32 inner operator: orders from level_order(); FP64 outer applies: fine
+    // below 8 (R_s + R_t), then orders 9, 7, 6 from 8, 16, 32 (R_s + R_t).
+    ls32_.alo[0] = 4.0, ls32_.alo[1] = 8.0, ls32_.alo[2] = 14.0;
+    ls32_.order[0] = ls32_.order[1] = ls32_.order[2] = -1;
+    ls32_.fp64 = false;
+    ls64_.alo[0] = 8.0, ls64_.alo[1] = 16.0, ls64_.alo[2] = 32.0;
+    ls64_.order[0] = 9, ls64_.order[1] = 7, ls64_.order[2] = 6;
+    ls64_.fp64 = true;
+    if (const char *v = std::getenv("SPHSTK_LEVEL_ALPHA"))
+        if (std::sscanf(v, "%lf,%lf,%lf", &ls32_.alo[0], &ls32_.alo[1], &ls32_.alo[2]) != 3)
+            throw Error(-1, "SPHSTK_LEVEL_ALPHA: a0,a1,a2");
+    if (const char *v = std::getenv("SPHSTK_OUTER_LEVELS"))
+        if (std::sscanf(v, "%lf,%lf,%lf:%d,%d,%d", &ls64_.alo[0], &ls64_.alo[1], &ls64_.alo[2], &ls64_.order[0],
+                        &ls64_.order[1], &ls64_.order[2]) != 6)
+            throw Error(-1, "SPHSTK_OUTER_LEVELS: a0,a1,a2:p0,p1,p2");
+    for (const LevelSet *S : {&ls32_, &ls64_}) {
+        SPHSTK_REQUIRE(S->alo[0] >= 1.5 && S->alo[1] > S->alo[0] && S->alo[2] > S->alo[1],
+                       "level separations must increase from >= 1.5");
+        for (int l = 0; l < kNumLevels; l++)
+            SPHSTK_REQUIRE(S->order[l] == -1 || (S->order[l] >= 1 && S->order[l] <= kMaxOrder),
+                           "level orders must lie in [1, max order]");
+    }
     multi_level_ = (flags_ & 2) == 0;
     if (const char *v = std::getenv("SPHSTK_MULTI_LEVEL")) multi_level_ = std::atoi(v) != 0;
+    if (const char *v = std::getenv("SPHSTK_FAR_MFMA")) far_mfma_ = std::atoi(v) != 0;
+    if (far_mfma_) tpl = 1;
     {
         std::vector<std::vector<int>> excl(nsph_);
         for (int t = 0; t < nsph_; t++) {
@@ -247,7 +271,8 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
             for (int64_t k = near_off_[t]; k < near_off_[t + 1]; k++) excl[t].push_back(near_src_[k]);
             std::sort(excl[t].begin(), excl[t].end());
         }
-        build_far_lists(lists_fine_, nsusp_, susp_sph_off_, pt_off_, pt_off_, excl, false, tpl, split, kWantItems);
+        build_far_lists(lists_fine_, nsusp_, susp_sph_off_, pt_off_, pt_off_, excl, false, tpl, split, kWantItems,
+                        far_mfma_ ? pos_.data() : nullptr);
     }
 
     src64_.alloc(std::max<int64_t>(8 * npts_, 8));
@@ -277,12 +302,16 @@ Operator::~Operator() {
 // reference removes again after the FMM, cpp:332-348, :408-433, :641-690); in
 // include mode only the listed spheres.  Each group's list is cut into `split`
 // slices of nearly equal length, one work item each, launched in order of
-// decreasing work.
+// decreasing work.  With `sph_pos` (sphere centres) the groups are aligned to
+// spheres instead (at most 64 points of one sphere each, so every interval
+// applies to all lanes) and each group records its sphere's centre.
 void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sph_off,
                      const std::vector<int64_t> &toff, const std::vector<int64_t> &soff,
                      const std::vector<std::vector<int>> &lists, bool include, int tpl, int split,
-                     int64_t want_items) {
+                     int64_t want_items, const double *sph_pos) {
     SPHSTK_REQUIRE(tpl == 1 || tpl == 2 || tpl == 4, "targets per lane must be 1, 2 or 4");
+    const bool aligned = sph_pos != nullptr;
+    SPHSTK_REQUIRE(!aligned || tpl == 1, "sphere-aligned far-field lists use one target per lane");
     SPHSTK_REQUIRE(split >= 0 && split <= 64, "source split must be in [0, 64] (0 = automatic)");
     const int nsph = (int)toff.size() - 1;
     const int64_t ntrg = toff[nsph];
@@ -292,8 +321,12 @@ void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sp
     const int G = 64 * tpl;
     int64_t nbase = 0;
     for (int b = 0; b < nsusp; b++) {
-        const int64_t np = toff[susp_sph_off[b + 1]] - toff[susp_sph_off[b]];
-        nbase += (np + G - 1) / G;
+        if (aligned) {
+            for (int64_t T = susp_sph_off[b]; T < susp_sph_off[b + 1]; T++) nbase += (toff[T + 1] - toff[T] + G - 1) / G;
+        } else {
+            const int64_t np = toff[susp_sph_off[b + 1]] - toff[susp_sph_off[b]];
+            nbase += (np + G - 1) / G;
+        }
     }
     if (split == 0) {
         const int64_t nb = std::max<int64_t>(nbase, 1);
@@ -305,6 +338,7 @@ void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sp
     std::vector<FarGroup> groups;
     std::vector<FarInterval> ivs, giv;
     std::vector<int64_t> work;
+    std::vector<double> cen;
     double pairs = 0.0;
     Mask full;
     full.fill(~0ull);
@@ -312,8 +346,9 @@ void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sp
     for (int b = 0; b < nsusp; b++) {
         const int s0 = (int)susp_sph_off[b], s1 = (int)susp_sph_off[b + 1];
         const int64_t p0 = toff[s0], p1 = toff[s1];
-        for (int64_t t0 = p0; t0 < p1; t0 += G) {
-            const int cnt = (int)std::min<int64_t>(G, p1 - t0);
+        for (int64_t t0 = p0; t0 < p1;) {
+            const int64_t tend = aligned ? toff[tsph[t0] + 1] : p1;
+            const int cnt = (int)std::min<int64_t>(G, tend - t0);
             Mask act{};
             for (int i = 0; i < cnt; i++) act[i / 64] |= 1ull << (i % 64);
             spec.clear();
@@ -390,7 +425,13 @@ void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sp
                 g.iv_end = (int64_t)ivs.size();
                 groups.push_back(g);
                 work.push_back(quota);
+                if (aligned) {
+                    const int T = tsph[t0];
+                    for (int c = 0; c < 3; c++) cen.push_back(sph_pos[3 * T + c]);
+                    cen.push_back(0.0);
+                }
             }
+            t0 += cnt;
         }
     }
     for (const FarGroup &g : groups) {
@@ -400,9 +441,13 @@ void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sp
         SPHSTK_REQUIRE(g.tstart >= tlo && g.tstart + g.tcount <= thi && g.tcount > 0 && g.tcount <= G &&
                            g.slot >= 0 && g.slot < split,
                        "internal: far-field group out of range");
-        for (int64_t k = g.iv_begin; k < g.iv_end; k++)
+        for (int64_t k = g.iv_begin; k < g.iv_end; k++) {
             SPHSTK_REQUIRE(ivs[k].begin >= slo && ivs[k].end <= shi && ivs[k].begin < ivs[k].end,
                            "internal: far-field interval out of range");
+            SPHSTK_REQUIRE(!aligned || ivs[k].mask[0] == ~0ull, "internal: masked interval in an aligned list");
+        }
+        SPHSTK_REQUIRE(!aligned || tsph[g.tstart] == tsph[g.tstart + g.tcount - 1],
+                       "internal: aligned far-field group spans spheres");
     }
     std::vector<int> ord(groups.size());
     std::iota(ord.begin(), ord.end(), 0);
@@ -412,6 +457,8 @@ void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sp
     L.groups.upload(groups);
     L.iv.upload(ivs);
     L.order.upload(ord);
+    L.aligned = aligned;
+    if (aligned) L.cen.upload(cen);
     if (split > 1) L.part.alloc(size_t(split) * 3 * std::max<int64_t>(ntrg, 1));
 }
 
@@ -441,10 +488,6 @@ static std::vector<Chunk> make_chunks(int nsusp, const std::vector<int64_t> &sus
 // stored in Morton order of the sphere centres (per suspension) so that the
 // spatially compact exclusion sets of the last level are few index ranges.
 namespace {
-struct LevelSpec {
-    double alo, ahi;
-};
-constexpr int kNumLevels = 3;
 int level_order(int lvl, int p) {
     int pc = p;
     if (lvl == 0) pc = p < 10 ? p : p / 2 + 1;
@@ -466,14 +509,10 @@ uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
 }
 } // namespace
 
-void Operator::ensure_levels() {
-    if (levels_ready_) return;
-    // separations (in units of R_s + R_t) where levels 0, 1, 2 start
-    double a[kNumLevels] = {4.0, 8.0, 14.0};
-    if (const char *v = std::getenv("SPHSTK_LEVEL_ALPHA"))
-        if (std::sscanf(v, "%lf,%lf,%lf", &a[0], &a[1], &a[2]) != 3) throw Error(-1, "SPHSTK_LEVEL_ALPHA: a0,a1,a2");
-    SPHSTK_REQUIRE(a[0] >= 1.5 && a[1] > a[0] && a[2] > a[1], "level separations must increase from >= 1.5");
-    const LevelSpec kLevels[kNumLevels] = {{a[0], a[1]}, {a[1], a[2]}, {a[2], 1e300}};
+void Operator::ensure_levels(LevelSet &S) {
+    if (S.ready) return;
+    const double *a = S.alo;
+    auto lorder = [&](int l, int p) { return S.order[l] < 0 ? level_order(l, p) : std::min(p, S.order[l]); };
     // Morton order of the spheres within each suspension
     std::vector<int> perm(nsph_), rank(nsph_);
     for (int b = 0; b < nsusp_; b++) {
@@ -499,7 +538,7 @@ void Operator::ensure_levels() {
         }
     }
     // neighbour classes: mid (fine), shells of levels 0..L-2, exclusion list of the last level
-    const double amax = kLevels[kNumLevels - 1].alo;
+    const double amax = a[kNumLevels - 1];
     std::vector<std::vector<int>> mid(nsph_);
     std::vector<std::vector<std::vector<int>>> shell(kNumLevels, std::vector<std::vector<int>>(nsph_));
     for (int b = 0; b < nsusp_; b++) {
@@ -531,27 +570,27 @@ void Operator::ensure_levels() {
                             const double sep = std::sqrt(ddx * ddx + ddy * ddy + ddz * ddz) / (rad_[s] + rad_[t]);
                             if (s == t || sep < amax) shell[kNumLevels - 1][rank[t]].push_back(rank[s]);
                             if (s == t || sep < 1.5) continue;
-                            if (sep < kLevels[0].alo) {
+                            if (sep < a[0]) {
                                 mid[t].push_back(s);
                                 continue;
                             }
                             for (int l = 0; l + 1 < kNumLevels; l++)
-                                if (sep >= kLevels[l].alo && sep < kLevels[l].ahi) shell[l][rank[t]].push_back(rank[s]);
+                                if (sep >= a[l] && sep < a[l + 1]) shell[l][rank[t]].push_back(rank[s]);
                         }
                     }
             std::sort(mid[t].begin(), mid[t].end());
             for (int l = 0; l < kNumLevels; l++) std::sort(shell[l][rank[t]].begin(), shell[l][rank[t]].end());
         }
     }
-    build_far_lists(lists_mid_, nsusp_, susp_sph_off_, pt_off_, pt_off_, mid, true, tpl_, 0, kWantItems);
-    levels_.clear();
+    build_far_lists(S.mid, nsusp_, susp_sph_off_, pt_off_, pt_off_, mid, true, tpl_, 0, kWantItems);
+    S.levels.clear();
     for (int l = 0; l < kNumLevels; l++) {
         auto L = std::make_unique<FarLevel>();
         std::vector<int64_t> off(nsph_ + 1, 0), off_by_sph(nsph_);
         int maxpts = 0;
         for (int r = 0; r < nsph_; r++) {
             const int s = perm[r];
-            const int p = order_[s], pc = level_order(l, p);
+            const int p = order_[s], pc = lorder(l, p);
             off[r + 1] = off[r] + grid_points(pc);
             off_by_sph[s] = off[r];
             maxpts = std::max(maxpts, std::max(grid_points(p), grid_points(pc)));
@@ -559,7 +598,7 @@ void Operator::ensure_levels() {
         const int64_t nc = off[nsph_];
         std::vector<double> cc(3 * nc), cn(3 * nc), cw(nc);
         for (int s = 0; s < nsph_; s++) {
-            const ShTables &t = sh_tables(level_order(l, order_[s]));
+            const ShTables &t = sh_tables(lorder(l, order_[s]));
             for (int q = 0; q < t.npts; q++) {
                 const int64_t g = off_by_sph[s] + q;
                 double nv[3];
@@ -581,7 +620,7 @@ void Operator::ensure_levels() {
         for (int p : order_) used[p] = true;
         for (int p = 0; p <= kMaxOrder; p++)
             if (used[p]) {
-                const Resample &rs = resample_tables(p, level_order(l, p));
+                const Resample &rs = resample_tables(p, lorder(l, p));
                 L->d_down[p].upload(rs.down);
                 L->d_up[p].upload(rs.up);
                 L->down.M[p] = L->d_down[p].p;
@@ -597,12 +636,16 @@ void Operator::ensure_levels() {
         L->chunks.upload(make_chunks(nsusp_, susp_sph_off_, off, &dummy));
         L->sv.alloc(std::max<int64_t>(3 * nc, 1));
         L->tv.alloc(std::max<int64_t>(3 * nc, 1));
-        L->src32.alloc(std::max<int64_t>(8 * nc, 8));
-        L->trg32.alloc(std::max<int64_t>(8 * nc, 8));
-        launch_pack_trg_f32(nc, L->coords.p, L->norms.p, L->trg32.p, stream_);
-        levels_.push_back(std::move(L));
+        if (S.fp64) {
+            L->src64.alloc(std::max<int64_t>(8 * nc, 8));
+        } else {
+            L->src32.alloc(std::max<int64_t>(8 * nc, 8));
+            L->trg32.alloc(std::max<int64_t>(8 * nc, 8));
+            launch_pack_trg_f32(nc, L->coords.p, L->norms.p, L->trg32.p, stream_);
+        }
+        S.levels.push_back(std::move(L));
     }
-    levels_ready_ = true;
+    S.ready = true;
 }
 
 void Operator::harvest_events() {
@@ -620,14 +663,66 @@ void Operator::harvest_events() {
 void Operator::far_run(FarLists &L, int kind, int prec, const void *src, const double *tcoords,
                        const double *tnorms, const float *trg32, const Chunk *chunks, int nchunks, double coef,
                        double *out, const int *active) {
-    if (L.split > 1) {
+    const bool mfma = kind == 0 && prec == 0 && L.aligned && far_mfma_;
+    double *dst = L.split > 1 ? L.part.p : out;
+    const int acc = L.split > 1 ? 0 : 1;
+    if (mfma)
+        launch_far_trac_mfma_f64((const double *)src, tcoords, tnorms, L.cen.p, L.ntrg, L.ngroups, L.order.p,
+                                 L.groups.p, L.iv.p, active, dst, coef, acc, stream_);
+    else
         launch_far(kind, prec, L.tpl, src, tcoords, tnorms, trg32, L.ntrg, L.ngroups, L.order.p, L.groups.p, L.iv.p,
-                   active, L.part.p, coef, 0, stream_);
-        launch_slot_reduce(out, L.part.p, L.split, 3 * L.ntrg, chunks, nchunks, active, stream_);
-    } else {
-        launch_far(kind, prec, L.tpl, src, tcoords, tnorms, trg32, L.ntrg, L.ngroups, L.order.p, L.groups.p, L.iv.p,
-                   active, out, coef, 1, stream_);
+                   active, dst, coef, acc, stream_);
+    if (L.split > 1) launch_slot_reduce(out, L.part.p, L.split, 3 * L.ntrg, chunks, nchunks, active, stream_);
+}
+
+// Multi-level far field (the packed fine-grid sources of `in` are already in
+// src64_ or src32_): each level resamples `in` to its grid, sums its shell of
+// sphere pairs there and interpolates the result back onto the fine grid; the
+// pairs closer than the first level are summed on the fine grid.  ev_a and
+// ev_b bracket the pair sums only.
+void Operator::far_levels(LevelSet &S, int kind, double coef, const double *in, double *out, const int *active,
+                          hipEvent_t ev_a, hipEvent_t ev_b) {
+    ensure_levels(S);
+    const int prec = S.fp64 ? 0 : 1;
+    for (auto &L : S.levels) {
+        launch_resample(geom_, L->down, d_pt_off_.p, L->d_off_by_sph.p, in, L->sv.p, 0, active, stream_, L->max_pts);
+        if (S.fp64)
+            launch_pack_src_f64(L->npts, L->coords.p, L->w.p, L->sv.p, L->src64.p, stream_);
+        else
+            launch_pack_src_f32(L->npts, L->coords.p, L->w.p, L->sv.p, L->src32.p, stream_);
+        launch_fill(3 * L->npts, 0.0, L->tv.p, stream_);
     }
+    if (ev_a) SPHSTK_HIP(hipEventRecord(ev_a, stream_));
+    for (auto &L : S.levels) {
+        if (S.fp64)
+            far_run(L->lists, kind, 0, L->src64.p, L->coords.p, L->norms.p, nullptr, L->chunks.p, (int)L->chunks.n,
+                    coef, L->tv.p, active);
+        else
+            far_run(L->lists, kind, 1, L->src32.p, nullptr, nullptr, L->trg32.p, L->chunks.p, (int)L->chunks.n, coef,
+                    L->tv.p, active);
+    }
+    if (S.fp64)
+        far_run(S.mid, kind, prec, src64_.p, d_coords_.p, d_norms_.p, nullptr, chunks_.p, (int)chunks_.n, coef, out,
+                active);
+    else
+        far_run(S.mid, kind, prec, src32_.p, nullptr, nullptr, trg32_.p, chunks_.p, (int)chunks_.n, coef, out,
+                active);
+    if (ev_b) SPHSTK_HIP(hipEventRecord(ev_b, stream_));
+    for (auto &L : S.levels)
+        launch_resample(geom_, L->up, L->d_off_by_sph.p, d_pt_off_.p, L->tv.p, out, 1, active, stream_, L->max_pts);
+}
+
+// The FP64 multi-level far field is used when its pair evaluations (fine mid
+// range plus all levels) are at most 3/4 of the direct sum's; below that the
+// resampling and the extra launches eat the saving (many small suspensions).
+bool Operator::outer_levels_pay() {
+    if (outer_pay_ < 0) {
+        ensure_levels(ls64_);
+        double ml = ls64_.mid.pairs;
+        for (auto &L : ls64_.levels) ml += L->lists.pairs;
+        outer_pay_ = ml <= 0.75 * lists_fine_.pairs ? 1 : 0;
+    }
+    return outer_pay_ == 1;
 }
 
 // Far field of `in` into out (accumulating).  FP64: fine grid, direct sum over
@@ -652,6 +747,19 @@ void Operator::far(int kind, int prec, double coef, const double *in, double *ou
         if (timing) SPHSTK_HIP(hipEventRecord(ev.a, stream_));
         far_run(lists_fine_, kind, 0, src64_.p, d_coords_.p, d_norms_.p, nullptr, chunks_.p, (int)chunks_.n, coef,
                 out, active);
+    } else if (prec == 2 && !outer_levels_pay()) {
+        // the coarse levels would save too few pair evaluations: direct FP64 sum
+        launch_pack_src_f64(npts_, d_coords_.p, d_w_.p, in, src64_.p, stream_);
+        if (timing) SPHSTK_HIP(hipEventRecord(ev.a, stream_));
+        far_run(lists_fine_, kind, 0, src64_.p, d_coords_.p, d_norms_.p, nullptr, chunks_.p, (int)chunks_.n, coef,
+                out, active);
+    } else if (prec == 2) {
+        launch_pack_src_f64(npts_, d_coords_.p, d_w_.p, in, src64_.p, stream_);
+        far_levels(ls64_, kind, coef, in, out, active, timing ? ev.a : nullptr, timing ? ev.b : nullptr);
+        if (timing) {
+            pending_.push_back(ev);
+            timing_done_ = true;
+        }
     } else {
         if (!trg32_.p) {
             trg32_.alloc(8 * std::max<int64_t>(npts_, 1));
@@ -660,27 +768,11 @@ void Operator::far(int kind, int prec, double coef, const double *in, double *ou
         }
         launch_pack_src_f32(npts_, d_coords_.p, d_w_.p, in, src32_.p, stream_);
         if (multi_level_) {
-            ensure_levels();
-            for (auto &L : levels_) {
-                launch_resample(geom_, L->down, d_pt_off_.p, L->d_off_by_sph.p, in, L->sv.p, 0, active, stream_,
-                                L->max_pts);
-                launch_pack_src_f32(L->npts, L->coords.p, L->w.p, L->sv.p, L->src32.p, stream_);
-                launch_fill(3 * L->npts, 0.0, L->tv.p, stream_);
-            }
-            if (timing) SPHSTK_HIP(hipEventRecord(ev.a, stream_));
-            for (auto &L : levels_)
-                far_run(L->lists, kind, 1, L->src32.p, nullptr, nullptr, L->trg32.p, L->chunks.p, (int)L->chunks.n,
-                        coef, L->tv.p, active);
-            far_run(lists_mid_, kind, 1, src32_.p, nullptr, nullptr, trg32_.p, chunks_.p, (int)chunks_.n, coef, out,
-                    active);
+            far_levels(ls32_, kind, coef, in, out, active, timing ? ev.a : nullptr, timing ? ev.b : nullptr);
             if (timing) {
-                SPHSTK_HIP(hipEventRecord(ev.b, stream_));
                 pending_.push_back(ev);
                 timing_done_ = true;
             }
-            for (auto &L : levels_)
-                launch_resample(geom_, L->up, L->d_off_by_sph.p, d_pt_off_.p, L->tv.p, out, 1, active, stream_,
-                                L->max_pts);
         } else {
             if (timing) SPHSTK_HIP(hipEventRecord(ev.a, stream_));
             far_run(lists_fine_, kind, 1, src32_.p, nullptr, nullptr, trg32_.p, chunks_.p, (int)chunks_.n, coef, out,
@@ -722,20 +814,72 @@ void Operator::p2p(const double *in, double *out, double cId, double cSL, double
     SPHSTK_HIP(hipGetLastError());
 }
 
+// Other spheres of the same suspension with centre distance below
+// amax (R_s + R_t), per sphere, sorted.
+std::vector<std::vector<int>> Operator::neighbours_within(double amax) const {
+    std::vector<std::vector<int>> nb(nsph_);
+    for (int b = 0; b < nsusp_; b++) {
+        const int s0 = (int)susp_sph_off_[b], s1 = (int)susp_sph_off_[b + 1];
+        if (s1 <= s0) continue;
+        double rmax = 0, lo[3] = {1e300, 1e300, 1e300};
+        for (int i = s0; i < s1; i++) {
+            rmax = std::max(rmax, rad_[i]);
+            for (int c = 0; c < 3; c++) lo[c] = std::min(lo[c], pos_[3 * i + c]);
+        }
+        const double h = 2.0 * amax * rmax;
+        std::unordered_map<int64_t, std::vector<int>> cells;
+        auto key = [](int64_t a, int64_t bb, int64_t c) { return (a * 1000003LL + bb) * 1000033LL + c; };
+        std::vector<std::array<int64_t, 3>> cid(s1 - s0);
+        for (int i = s0; i < s1; i++) {
+            for (int c = 0; c < 3; c++) cid[i - s0][c] = (int64_t)std::floor((pos_[3 * i + c] - lo[c]) / h);
+            cells[key(cid[i - s0][0], cid[i - s0][1], cid[i - s0][2])].push_back(i);
+        }
+        for (int t = s0; t < s1; t++) {
+            const auto &ct = cid[t - s0];
+            for (int dx = -1; dx <= 1; dx++)
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dz = -1; dz <= 1; dz++) {
+                        auto it = cells.find(key(ct[0] + dx, ct[1] + dy, ct[2] + dz));
+                        if (it == cells.end()) continue;
+                        for (int s : it->second) {
+                            const double ex = pos_[3 * s] - pos_[3 * t], ey = pos_[3 * s + 1] - pos_[3 * t + 1],
+                                         ez = pos_[3 * s + 2] - pos_[3 * t + 2];
+                            if (s != t && std::sqrt(ex * ex + ey * ey + ez * ez) < amax * (rad_[s] + rad_[t]))
+                                nb[t].push_back(s);
+                        }
+                    }
+            std::sort(nb[t].begin(), nb[t].end());
+        }
+    }
+    return nb;
+}
+
 // Rigid-body averages of u = S[rho] (V = mean u / mu, Omega = 3/(8 pi R^4 mu) sum w R n x u):
 // self and near terms by quadrature of the SH evaluations on the grid (as the
-// reference), far pairs through the sphere-centre sums (Faxen's laws).
+// reference), far pairs closer than kFaxenSep (R_s + R_t) by the direct FP64
+// sum on the grid and the same quadrature, farther pairs through the
+// sphere-centre sums (Faxen's laws: the exact surface averages, which the grid
+// quadrature of order p reproduces to ~(R_t / d)^(2p+2), below 1e-10 for p >= 6
+// beyond 4 (R_s + R_t) when R_t / (R_s + R_t) <= 0.75).
+constexpr double kFaxenSep = 4.0;
 void Operator::rigid_velocity(const double *rho, double inv_mu, double *vel) {
     if (nsph_ == 0) return;
     if (!centre_ready_) {
         std::vector<int64_t> toff(nsph_ + 1);
         for (int i = 0; i <= nsph_; i++) toff[i] = i;
-        std::vector<std::vector<int>> excl(nsph_);
+        // excluded from the centre sums: the sphere itself and every sphere
+        // closer than kFaxenSep; of those, the non-near ones form the grid list
+        std::vector<std::vector<int>> excl = neighbours_within(kFaxenSep), grid_mid(nsph_);
         for (int t = 0; t < nsph_; t++) {
+            for (int s : excl[t]) {
+                const double ex = pos_[3 * s] - pos_[3 * t], ey = pos_[3 * s + 1] - pos_[3 * t + 1],
+                             ez = pos_[3 * s + 2] - pos_[3 * t + 2];
+                if (std::sqrt(ex * ex + ey * ey + ez * ez) >= 1.5 * (rad_[s] + rad_[t])) grid_mid[t].push_back(s);
+            }
             excl[t].push_back(t);
-            for (int64_t k = near_off_[t]; k < near_off_[t + 1]; k++) excl[t].push_back(near_src_[k]);
             std::sort(excl[t].begin(), excl[t].end());
         }
+        build_far_lists(lists_rv_mid_, nsusp_, susp_sph_off_, pt_off_, pt_off_, grid_mid, true, tpl_, 0, kWantItems);
         int64_t nb = 0;
         for (int b = 0; b < nsusp_; b++) nb += (susp_sph_off_[b + 1] - susp_sph_off_[b] + 63) / 64;
         const int split = (int)std::max<int64_t>(1, std::min<int64_t>(64, (kWantItems + nb - 1) / std::max<int64_t>(nb, 1)));
@@ -750,8 +894,10 @@ void Operator::rigid_velocity(const double *rho, double inv_mu, double *vel) {
     launch_fill(3 * npts_, 0.0, u_ns_.p, stream_);
     launch_sheval(geom_, 0, rec_.p, items_.p, (int)items_.n, d_trg_slot_off_.p, d_trg_nsrc_.p, evscratch_.p, rho,
                   u_ns_.p, 1.0, 0.0, 0.0, nullptr, stream_);
-    launch_rigid_avg(geom_, u_ns_.p, inv_mu, vel_ns_.p, stream_);
     launch_pack_src_f64(npts_, d_coords_.p, d_w_.p, rho, src64_.p, stream_);
+    far_run(lists_rv_mid_, 1, 0, src64_.p, d_coords_.p, d_norms_.p, nullptr, chunks_.p, (int)chunks_.n,
+            1.0 / (8.0 * kPi), u_ns_.p, nullptr);
+    launch_rigid_avg(geom_, u_ns_.p, inv_mu, vel_ns_.p, stream_);
     launch_far_centre(src64_.p, d_pos_.p, lists_centre_.ngroups, lists_centre_.order.p, lists_centre_.groups.p,
                       lists_centre_.iv.p, centre_part_.p, nsph_, stream_);
     launch_centre_finish(nsph_, lists_centre_.split, centre_part_.p, d_rad_.p, vel_ns_.p, inv_mu, vel, stream_);
@@ -803,17 +949,18 @@ void Operator::near_pairs(int32_t *trg, int32_t *src) const {
 
 void Operator::kernel_stats(int kind, int prec, double *ms, int64_t *launches, double *pairs) {
     SPHSTK_REQUIRE(kind == 0 || kind == 1, "kind must be 0 (traction) or 1 (single layer)");
-    SPHSTK_REQUIRE(prec == 0 || prec == 1, "prec must be 0 (FP64) or 1 (FP32)");
+    SPHSTK_REQUIRE(prec >= 0 && prec < kNumPrec, "prec must be 0 (FP64), 1 (FP32) or 2 (FP64 multi-level)");
     harvest_events();
     if (ms) *ms = stats_[kind][prec].ms;
     if (launches) *launches = stats_[kind][prec].launches;
     if (pairs) {
-        if (prec == 1 && multi_level_ && levels_ready_) {
-            *pairs = lists_mid_.pairs;
-            for (auto &L : levels_) *pairs += L->lists.pairs;
-        }
-        else
+        const LevelSet *S = (prec == 2 && outer_pay_ == 1) ? &ls64_ : (prec == 1 && multi_level_) ? &ls32_ : nullptr;
+        if (S && S->ready) {
+            *pairs = S->mid.pairs;
+            for (auto &L : S->levels) *pairs += L->lists.pairs;
+        } else {
             *pairs = lists_fine_.pairs;
+        }
     }
 }
 
@@ -829,6 +976,7 @@ void Operator::reset_stats() {
 Mobility::Mobility(int nsusp, const int *counts, const double *pos, const double *radius, const double *quat,
                    const int *order, double viscosity, const SolverParams &prm, int precision, int flags)
     : mu_(viscosity), prm_(prm), prec_(precision) {
+    if (const char *v = std::getenv("SPHSTK_OUTER_ML")) outer_prec_ = std::atoi(v) != 0 ? 2 : 0;
     SPHSTK_REQUIRE(std::isfinite(viscosity) && viscosity > 0, "viscosity must be > 0");
     SPHSTK_REQUIRE(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (mixed)");
     if (const char *v = std::getenv("SPHSTK_VERBOSE")) verbose_ = std::atoi(v) != 0;
@@ -1050,7 +1198,8 @@ void Mobility::solve(const double *force, double *vel, std::vector<SegInfo> &inf
     launch_rho0(A.geom(), force_.p, rho_.p, st);
     // b = -(1/2 I + K') rho0   (:196)
     launch_fill(n, 0.0, b_.p, st);
-    A.p2p(rho_.p, b_.p, -0.5, 0.0, -1.0, 0, nullptr);
+    const int p64 = prec_ == 0 ? 0 : outer_prec_; // FP64 direct, or FP64 multi-level in the mixed form
+    A.p2p(rho_.p, b_.p, -0.5, 0.0, -1.0, p64, nullptr);
     app64++;
     std::vector<double> bn;
     seg_norms(b_.p, bn);
@@ -1083,7 +1232,7 @@ void Mobility::solve(const double *force, double *vel, std::vector<SegInfo> &inf
         if (x_zero_) {
             launch_axpby(n, 1.0, b_.p, 0.0, r_.p, st);
         } else {
-            A.apply_into(x_.p, r_.p, 0, nullptr);
+            A.apply_into(x_.p, r_.p, p64, nullptr);
             app64++;
             launch_axpby(n, 1.0, b_.p, -1.0, r_.p, st);
         }
@@ -1114,7 +1263,7 @@ void Mobility::solve(const double *force, double *vel, std::vector<SegInfo> &inf
             std::vector<double> ones(B, 1.0);
             SPHSTK_HIP(hipMemcpyAsync(scal_.p + 2 * B, ones.data(), sizeof(double) * B, hipMemcpyHostToDevice, st));
             launch_seg_axpy(x_.p, d_.p, scal_.p + 2 * B, 1.0, A.chunks(), A.nchunks(), act_.p, st);
-            A.apply_into(x_.p, r_.p, 0, act_.p);
+            A.apply_into(x_.p, r_.p, p64, act_.p);
             app64++;
             launch_axpby(n, 1.0, b_.p, -1.0, r_.p, st);
             seg_norms(r_.p, rn);

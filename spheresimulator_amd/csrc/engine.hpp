@@ -42,6 +42,8 @@ struct FarLists {
     DevBuf<FarInterval> iv;
     DevBuf<int> order;
     DevBuf<double> part; // split partial-sum slots of 3*ntrg doubles
+    DevBuf<double> cen;  // sphere-aligned lists: centre of each group's target sphere (4 per group)
+    bool aligned = false;
     int ngroups = 0, tpl = 1, split = 1;
     double pairs = 0.0; // algorithmic pairs per launch
     int64_t ntrg = 0;
@@ -50,7 +52,7 @@ struct FarLists {
 void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sph_off,
                      const std::vector<int64_t> &toff, const std::vector<int64_t> &soff,
                      const std::vector<std::vector<int>> &lists, bool include, int tpl, int split,
-                     int64_t want_items);
+                     int64_t want_items, const double *sph_pos = nullptr);
 
 // One level of the multi-level FP32 far field: per-sphere grids of a lower
 // order, their geometry, resampling tables and work lists.
@@ -60,10 +62,23 @@ struct FarLevel {
     DevBuf<int64_t> d_off_by_sph; // first level point of each sphere
     DevBuf<double> coords, norms, w, sv, tv;
     DevBuf<float> src32, trg32;
+    DevBuf<double> src64;
     DevBuf<Chunk> chunks;
     DevBuf<double> d_down[kMaxOrder + 1], d_up[kMaxOrder + 1];
     ResampleTabs down, up;
     FarLists lists;
+};
+
+// A multi-level far field: fine-grid sums for the pairs closer than the first
+// level's separation (mid), then one coarser grid per separation shell.
+constexpr int kNumLevels = 3;
+struct LevelSet {
+    double alo[kNumLevels];  // separations (in units of R_s + R_t) where levels 0, 1, 2 start
+    int order[kNumLevels];   // grid order of each level (capped by the sphere's order)
+    bool fp64 = false;       // FP64 pair sums (else FP32)
+    std::vector<std::unique_ptr<FarLevel>> levels;
+    FarLists mid;
+    bool ready = false;
 };
 
 struct KernelStats {
@@ -103,7 +118,9 @@ class Operator {
 
     // device-pointer operations (all on stream()); `active` is a device array of
     // one int per suspension or nullptr for all.
-    // out += cId in + cSL S[in] + cTrac K'[in]   (far field in FP64 (prec 0) or FP32 (prec 1))
+    // out += cId in + cSL S[in] + cTrac K'[in].  Far field: prec 0 FP64 direct
+    // sum, 1 FP32 (multi-level unless disabled), 2 FP64 multi-level.
+    static constexpr int kNumPrec = 3;
     void p2p(const double *in, double *out, double cId, double cSL, double cTrac, int prec, const int *active);
     // out += cLOP L[in]
     void lop(const double *in, double *out, double cLOP, const int *active);
@@ -112,7 +129,7 @@ class Operator {
     // Y = beta Y + alpha A X
     void apply(const double *X, double *Y, double alpha, double beta, int prec);
     void project(double *inout);
-    // rigid-body velocities (6 per sphere) of the single layer of rho, far pairs by Faxen's laws
+    // rigid-body velocities (6 per sphere) of the single layer of rho, distant pairs by Faxen's laws
     void rigid_velocity(const double *rho, double inv_mu, double *vel);
 
     // host accessors
@@ -135,7 +152,13 @@ class Operator {
     void far(int kind, int prec, double coef, const double *in, double *out, const int *active);
     void far_run(FarLists &L, int kind, int prec, const void *src, const double *tcoords, const double *tnorms,
                  const float *trg32, const Chunk *chunks, int nchunks, double coef, double *out, const int *active);
-    void ensure_levels();
+    void ensure_levels(LevelSet &S);
+    std::vector<std::vector<int>> neighbours_within(double amax) const;
+    bool outer_levels_pay();
+    // multi-level far field of `in` into out; ev_a and ev_b (if set) bracket
+    // the pair-sum launches
+    void far_levels(LevelSet &S, int kind, double coef, const double *in, double *out, const int *active,
+                    hipEvent_t ev_a, hipEvent_t ev_b);
 
     int nsph_, nsusp_;
     int64_t npts_;
@@ -160,13 +183,15 @@ class Operator {
     DevGeom geom_;
 
     // far-field work lists: fine grid (FP64 and direct FP32); the multi-level
-    // FP32 far field (fine mid-range pairs + coarser levels)
-    FarLists lists_fine_, lists_mid_;
-    std::vector<std::unique_ptr<FarLevel>> levels_;
-    bool multi_level_ = true, levels_ready_ = false, timing_done_ = false;
+    // far fields of the FP32 inner operator and of the FP64 outer applies
+    FarLists lists_fine_;
+    LevelSet ls32_, ls64_;
+    bool multi_level_ = true, timing_done_ = false;
+    int outer_pay_ = -1; // FP64 multi-level outer far field in use: -1 undecided, 0 no, 1 yes
+    bool far_mfma_ = false; // FP64 traction far field with the dot products on the matrix cores (opt-in)
 
-    // sphere-centre far field for rigid_velocity
-    FarLists lists_centre_;
+    // rigid_velocity: fine-grid sums of the pairs closer than kFaxenSep, sphere-centre sums of the rest
+    FarLists lists_rv_mid_, lists_centre_;
     bool centre_ready_ = false;
     DevBuf<double> centre_part_, vel_ns_, u_ns_;
 
@@ -190,7 +215,7 @@ class Operator {
     };
     std::vector<EvPair> pending_;
     std::vector<EvPair> free_events_;
-    KernelStats stats_[2][2];
+    KernelStats stats_[2][kNumPrec];
     void harvest_events();
 };
 
@@ -219,6 +244,7 @@ class Mobility {
     double mu_;
     SolverParams prm_;
     int prec_;
+    int outer_prec_ = 2; // far field of the mixed form's FP64 applies: 2 multi-level, 0 direct
     bool x_zero_ = true; // the stored unknown is known to be zero (cold start)
     bool verbose_ = false; // SPHSTK_VERBOSE=1: residual history on stderr (Belos Verbosity 63)
     DevBuf<double> rho_, b_, x_, r_, d_, V_, h_, scal_, coefv_, force_, vel_, gmres_inner_rhs_;

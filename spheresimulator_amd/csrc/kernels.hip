@@ -15,6 +15,7 @@
 // LOP         k_lop          rigid-body projector (:487-521).
 // BLAS-1      k_seg_*        segmented (one segment per suspension) GMRES
 //                            vector kernels.
+#include "device_math.hpp"
 #include "kernels.hpp"
 
 namespace sphstk {
@@ -137,13 +138,23 @@ __device__ __forceinline__ void far_pair(const R sx, const R sy, const R sz, con
     for (int t = 0; t < TPL; t++) {
         const R dx = x[t] - sx, dy = y[t] - sy, dz = z[t] - sz;
         const R r2 = fmad(dx, dx, fmad(dy, dy, dz * dz));
+        const R df = fmad(dx, fx, fmad(dy, fy, dz * fz));
+        if constexpr (KIND == 0 && sizeof(R) == 8) {
+            R w = trac_w_f64(r2);
+            if (MASKED) w = incl[t] ? w : R(0);
+            const R dn = fmad(dx, nx[t], fmad(dy, ny[t], dz * nz[t]));
+            const R q = (df * dn) * w;
+            ax[t] = fmad(q, dx, ax[t]);
+            ay[t] = fmad(q, dy, ay[t]);
+            az[t] = fmad(q, dz, az[t]);
+            continue;
+        }
         R ri;
         if constexpr (sizeof(R) == 8)
             ri = rsqrt_f64(r2);
         else
             ri = __builtin_amdgcn_rsqf(r2);
         if (MASKED) ri = incl[t] ? ri : R(0);
-        const R df = fmad(dx, fx, fmad(dy, fy, dz * fz));
         if (KIND == 0) {
             const R dn = fmad(dx, nx[t], fmad(dy, ny[t], dz * nz[t]));
             const R ri2 = ri * ri;
@@ -248,7 +259,7 @@ __global__ __launch_bounds__(64) void k_far_f64(const double *__restrict__ src, 
         else
             far_range<KIND, TPL, true>(src, b, e, x, y, z, nx, ny, nz, incl, ax, ay, az);
     }
-    store_far<TPL>(G, lane, ax, ay, az, out, coef, accumulate, slot_stride);
+    store_far<TPL>(G, lane, ax, ay, az, out, KIND == 0 ? kTracW * coef : coef, accumulate, slot_stride);
 }
 
 // far field, FP32 pair arithmetic with two-level accumulation: FP32 partial sums

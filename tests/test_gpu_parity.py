@@ -217,3 +217,65 @@ def test_inner_fp32_operator_accuracy(p, direct, tol):
     err = rel(y32, y64)
     print("inner rel err %.3e" % err)
     assert err < tol
+
+
+def test_far_traction_mfma_path_matches():
+    """The opt-in matrix-core FP64 traction far field (SPHSTK_FAR_MFMA=1:
+    sphere-aligned work items, dot products on v_mfma_f64_16x16x4f64) agrees
+    with the oracle like the default vector path, on mixed orders and radii
+    (partial 16-target column tiles, interval tails)."""
+    import os
+    say("far traction MFMA path")
+    pos, rad, quat, order = small_geometry(seed=41, n=12, orders=(5, 6, 8, 7, 10))
+    ref = O.Operator(pos, rad, quat, order)
+    x = np.random.default_rng(12).standard_normal(ref.dof)
+    want = ref.apply_p2p(x, None, 0.0, 0.0, 1.0)
+    old = os.environ.get("SPHSTK_FAR_MFMA")
+    os.environ["SPHSTK_FAR_MFMA"] = "1"
+    try:
+        gpu = S.SphereSTKSHOperator(pos, rad, quat, order)
+    finally:
+        if old is None:
+            del os.environ["SPHSTK_FAR_MFMA"]
+        else:
+            os.environ["SPHSTK_FAR_MFMA"] = old
+    got = gpu.applyP2POP(x, np.zeros(ref.dof), 0.0, 0.0, 1.0)
+    assert relmax(got, want) < 1e-11
+
+
+@pytest.mark.parametrize("p", [10, 12])
+def test_outer_fp64_operator_accuracy(p):
+    """The FP64 multi-level outer operator of the mixed form (pairs beyond
+    8 (R_s + R_t) through grids of order min(p, 9/7/6)) matches the FP64 direct
+    operator on a band-limited density and on a white one (no band limit), to
+    far below the 1e-8 GMRES tolerance the refinement residual serves.  The
+    dilute suspension (vf 0.02) makes the levels pay, so they are in use."""
+    say("outer FP64 multi-level operator (p=%d)" % p)
+    c = S.configs.make_suspension(256, p, 33, vf=0.02, min_gap=0.2)
+    gpu = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+    white = np.random.default_rng(9).standard_normal(gpu.n_dof)
+    for name, x, tol in (("band-limited", gpu.projectNullSpace(white.copy()), 2e-11), ("white", white, 1e-10)):
+        err = rel(gpu.apply_outer(x), gpu.apply(x))
+        print("outer %s rel err %.3e" % (name, err))
+        assert err < tol
+    ml, direct = gpu.kernel_stats(0, 2)["pairs"], gpu.kernel_stats(0, 0)["pairs"]
+    print("pair evaluations: multi-level %.4g, direct %.4g" % (ml, direct))
+    assert 0 < ml < 0.75 * direct
+
+
+def test_mixed_matches_fp64_low_order_polydisperse():
+    """Mixed form vs FP64 form on polydisperse suspensions with low orders (the
+    first suspensions of the batched workload: p in {6..16}, R in [0.5, 1.5],
+    gaps 0.1 (R_s + R_t)): final velocities agree to 1e-8, i.e. the sphere-
+    centre (Faxen) sums are used only where the grid quadrature matches them."""
+    say("mixed vs FP64, polydisperse low order")
+    c = S.configs.config4(n_susp=6)
+    v = {}
+    for prec in (S.PREC_FP64, S.PREC_MIXED):
+        M = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], counts=c["counts"], precision=prec)
+        v[prec], info = M.solveMob(c["force"])
+        assert info["converged"] == 1
+    a, b = v[S.PREC_MIXED].reshape(-1, 6), v[S.PREC_FP64].reshape(-1, 6)
+    ev, ew = rel(a[:, :3], b[:, :3]), rel(a[:, 3:], b[:, 3:])
+    print("rel err V %.3e Omega %.3e" % (ev, ew))
+    assert ev < 1e-8 and ew < 1e-8
