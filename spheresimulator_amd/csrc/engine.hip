@@ -4,6 +4,10 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <functional>
+#include <thread>
+#include <chrono>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -19,6 +23,25 @@ namespace {
 constexpr int kMaxSupportedOrder = 20; // LDS of the expansion kernel bounds the order
 constexpr int64_t kChunk = 16384;
 constexpr int64_t kWantItems = 40000; // far-field work items wanted when a problem is small
+
+// SPHSTK_VERBOSE=1: set-up phase times on stderr
+static bool setup_verbose() {
+    static const bool v = [] {
+        const char *e = std::getenv("SPHSTK_VERBOSE");
+        return e && std::atoi(e) != 0;
+    }();
+    return v;
+}
+struct PhaseTimer {
+    const char *what;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    explicit PhaseTimer(const char *w) : what(w) {}
+    ~PhaseTimer() {
+        if (setup_verbose())
+            std::fprintf(stderr, "[sphstk] setup %s: %.3f s\n", what,
+                         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    }
+};
 
 inline void qrot_host(const double *q, const double *v, double *o) {
     const double w = q[0], ux = q[1], uy = q[2], uz = q[3];
@@ -42,6 +65,7 @@ static std::vector<Chunk> make_chunks(int nsusp, const std::vector<int64_t> &sus
 Operator::Operator(int nsusp, const int *counts, const double *pos, const double *radius, const double *quat,
                    const int *order, double cId, double cSL, double cTrac, double cLOP, int flags)
     : nsph_(0), nsusp_(nsusp), npts_(0), flags_(flags) {
+    PhaseTimer pt("operator");
     c_[0] = cId;
     c_[1] = cSL;
     c_[2] = cTrac;
@@ -219,6 +243,24 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     geom_.w = d_w_.p;
     geom_.near_off = d_near_off_.p;
     geom_.near_src = d_near_src_.p;
+    {
+        // spheres grouped by order, in index order within an order, tiles of kTileSph
+        std::vector<int> ts(nsph_);
+        std::iota(ts.begin(), ts.end(), 0);
+        std::stable_sort(ts.begin(), ts.end(), [&](int a, int b) { return order_[a] < order_[b]; });
+        std::vector<SphTile> tl;
+        for (int i = 0; i < nsph_;) {
+            int j = i;
+            while (j < nsph_ && j - i < kTileSph && order_[ts[j]] == order_[ts[i]]) j++;
+            tl.push_back(SphTile{order_[ts[i]], i, j - i, 0});
+            i = j;
+        }
+        d_tile_sph_.upload(ts);
+        d_tiles_.upload(tl);
+        geom_.ntiles = (int)tl.size();
+        geom_.tiles = d_tiles_.p;
+        geom_.tile_sph = d_tile_sph_.p;
+    }
     std::vector<bool> used(kMaxOrder + 1, false);
     for (int p : order_) used[p] = true;
     for (int p = 0; p <= kMaxOrder; p++)
@@ -265,13 +307,19 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     if (const char *v = std::getenv("SPHSTK_FAR_MFMA")) far_mfma_ = std::atoi(v) != 0;
     if (far_mfma_) tpl = 1;
     {
+        PhaseTimer pt2("fine far-field lists");
         std::vector<std::vector<int>> excl(nsph_);
         for (int t = 0; t < nsph_; t++) {
             excl[t].push_back(t);
             for (int64_t k = near_off_[t]; k < near_off_[t + 1]; k++) excl[t].push_back(near_src_[k]);
             std::sort(excl[t].begin(), excl[t].end());
         }
-        build_far_lists(lists_fine_, nsusp_, susp_sph_off_, pt_off_, pt_off_, excl, false, tpl, split, kWantItems,
+        std::vector<Runs> runs(nsph_);
+        for (int t = 0; t < nsph_; t++) {
+            const int b = sph_susp_[t];
+            runs[t] = runs_of(excl[t], (int)susp_sph_off_[b], (int)susp_sph_off_[b + 1], false);
+        }
+        build_far_lists(lists_fine_, nsusp_, susp_sph_off_, pt_off_, pt_off_, runs, tpl, split, kWantItems,
                         far_mfma_ ? pos_.data() : nullptr);
     }
 
@@ -294,26 +342,68 @@ Operator::~Operator() {
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+// Runs of the sorted sphere list `lst` within [s0, s1) (include) or of its
+// complement there (exclude).
+Runs runs_of(const std::vector<int> &lst, int s0, int s1, bool include) {
+    Runs r;
+    if (include) {
+        for (int s : lst)
+            if (s >= s0 && s < s1) run_append(r, s);
+        return r;
+    }
+    int cur = s0;
+    for (int s : lst) {
+        if (s < cur || s >= s1) continue;
+        if (s > cur) r.push_back({cur, s});
+        cur = s + 1;
+    }
+    if (cur < s1) r.push_back({cur, s1});
+    return r;
+}
+
+// Runs a parallel loop over [0, n) on the host's cores (at most 16 threads).
+static void parallel_for(int n, const std::function<void(int, int)> &body) {
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 256 || nt == 1) {
+        body(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    std::atomic<int> next{0};
+    const int chunk = std::max(16, n / (8 * nt));
+    for (int k = 0; k < nt; k++)
+        th.emplace_back([&] {
+            for (;;) {
+                const int a = next.fetch_add(chunk);
+                if (a >= n) break;
+                body(a, std::min(n, a + chunk));
+            }
+        });
+    for (auto &t : th) t.join();
+}
+
 // Far-field work lists.  Targets (a point set with per-sphere offsets `toff`)
 // are cut into groups of 64*tpl consecutive points of one suspension.  Each
-// group gets a list of source intervals over the point set `soff`: in exclude
-// mode all spheres of its suspension minus, per lane, the spheres listed for the
-// lane's target sphere (its own sphere and its near spheres: the pairs the
-// reference removes again after the FMM, cpp:332-348, :408-433, :641-690); in
-// include mode only the listed spheres.  Each group's list is cut into `split`
-// slices of nearly equal length, one work item each, launched in order of
-// decreasing work.  With `sph_pos` (sphere centres) the groups are aligned to
-// spheres instead (at most 64 points of one sphere each, so every interval
-// applies to all lanes) and each group records its sphere's centre.
+// target sphere T interacts with the source spheres of `runs[T]` (ranges of
+// sphere indices of the source point set `soff`, sorted, disjoint, within T's
+// suspension): for the direct far field all spheres of the suspension minus T
+// itself and its near spheres (the pairs the reference removes again after the
+// FMM, cpp:332-348, :408-433, :641-690).  A group's source intervals are the
+// point ranges of its spheres' runs, with a per-lane mask where the group
+// spans spheres.  Each group's list is cut into `split` slices of nearly equal
+// length, one work item each, launched in order of decreasing work.  With
+// `sph_pos` (sphere centres) the groups are aligned to spheres instead (at most
+// 64 points of one sphere each, so every interval applies to all lanes) and
+// each group records its sphere's centre.
 void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sph_off,
-                     const std::vector<int64_t> &toff, const std::vector<int64_t> &soff,
-                     const std::vector<std::vector<int>> &lists, bool include, int tpl, int split,
-                     int64_t want_items, const double *sph_pos) {
+                     const std::vector<int64_t> &toff, const std::vector<int64_t> &soff, const std::vector<Runs> &runs,
+                     int tpl, int split, int64_t want_items, const double *sph_pos) {
     SPHSTK_REQUIRE(tpl == 1 || tpl == 2 || tpl == 4, "targets per lane must be 1, 2 or 4");
+    SPHSTK_REQUIRE(split >= 0 && split <= 64, "source split must be in [0, 64] (0 = automatic)");
     const bool aligned = sph_pos != nullptr;
     SPHSTK_REQUIRE(!aligned || tpl == 1, "sphere-aligned far-field lists use one target per lane");
-    SPHSTK_REQUIRE(split >= 0 && split <= 64, "source split must be in [0, 64] (0 = automatic)");
     const int nsph = (int)toff.size() - 1;
+    SPHSTK_REQUIRE((int)runs.size() == nsph, "internal: one run list per target sphere");
     const int64_t ntrg = toff[nsph];
     std::vector<int32_t> tsph(ntrg);
     for (int i = 0; i < nsph; i++)
@@ -342,28 +432,39 @@ void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sp
     double pairs = 0.0;
     Mask full;
     full.fill(~0ull);
-    std::vector<std::pair<int, Mask>> spec;
+    std::vector<std::pair<int, Mask>> ev;
+    // groups inside one sphere all get that sphere's runs with full masks, so
+    // consecutive groups of the same sphere share one copy of the intervals
+    int last_T = -1;
+    int64_t last_work = 0;
+    std::vector<std::array<int64_t, 3>> last_slots; // iv_begin, iv_end, quota per slot
     for (int b = 0; b < nsusp; b++) {
-        const int s0 = (int)susp_sph_off[b], s1 = (int)susp_sph_off[b + 1];
-        const int64_t p0 = toff[s0], p1 = toff[s1];
+        const int64_t p0 = toff[susp_sph_off[b]], p1 = toff[susp_sph_off[b + 1]];
         for (int64_t t0 = p0; t0 < p1;) {
             const int64_t tend = aligned ? toff[tsph[t0] + 1] : p1;
             const int cnt = (int)std::min<int64_t>(G, tend - t0);
+            if (tsph[t0] == tsph[t0 + cnt - 1] && tsph[t0] == last_T) {
+                for (int sl = 0; sl < split; sl++) {
+                    groups.push_back(FarGroup{t0, last_slots[sl][0], last_slots[sl][1], cnt, b, sl, 0});
+                    work.push_back(last_slots[sl][2]);
+                    if (aligned) {
+                        for (int c = 0; c < 3; c++) cen.push_back(sph_pos[3 * last_T + c]);
+                        cen.push_back(0.0);
+                    }
+                }
+                pairs += double(last_work) * cnt;
+                t0 += cnt;
+                continue;
+            }
             Mask act{};
             for (int i = 0; i < cnt; i++) act[i / 64] |= 1ull << (i % 64);
-            spec.clear();
             giv.clear();
-            const int Ta = tsph[t0], Tb = tsph[t0 + cnt - 1];
-            for (int T = Ta; T <= Tb; T++) {
-                const int64_t lo = std::max(toff[T], t0) - t0, hi = std::min(toff[T + 1], t0 + cnt) - t0;
-                Mask lanes{};
-                for (int64_t i = lo; i < hi; i++) lanes[i / 64] |= 1ull << (i % 64);
-                for (int S : lists[T]) spec.push_back({S, lanes});
-            }
-            std::sort(spec.begin(), spec.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
             int64_t gwork = 0;
             auto emit = [&](int64_t bb, int64_t ee, const Mask &m) {
                 if (bb >= ee) return;
+                int lanes = 0;
+                for (int w = 0; w < tpl; w++) lanes += __builtin_popcountll(m[w] & act[w]);
+                if (lanes == 0) return;
                 if (!giv.empty() && giv.back().end == bb &&
                     std::memcmp(giv.back().mask, m.data(), sizeof(uint64_t) * kMaxTPL) == 0) {
                     giv.back().end = ee;
@@ -374,35 +475,50 @@ void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sp
                     std::memcpy(iv.mask, m.data(), sizeof(uint64_t) * kMaxTPL);
                     giv.push_back(iv);
                 }
-                int lanes = 0;
-                for (int w = 0; w < tpl; w++) lanes += __builtin_popcountll(m[w] & act[w]);
                 pairs += double(ee - bb) * lanes;
                 gwork += ee - bb;
             };
-            int cur = s0;
-            size_t k = 0;
-            while (k < spec.size()) {
-                const int S = spec[k].first;
-                Mask sel{};
-                while (k < spec.size() && spec[k].first == S) {
-                    for (int w = 0; w < kMaxTPL; w++) sel[w] |= spec[k].second[w];
-                    k++;
+            const int Ta = tsph[t0], Tb = tsph[t0 + cnt - 1];
+            if (Ta == Tb) {
+                for (const auto &r : runs[Ta]) emit(soff[r.first], soff[r.second], full);
+            } else {
+                // sweep over the run boundaries of the group's spheres; the lane
+                // sets of different spheres are disjoint, so toggling a sphere's
+                // lanes at its run boundaries gives the covering mask
+                ev.clear();
+                for (int T = Ta; T <= Tb; T++) {
+                    const int64_t lo = std::max(toff[T], t0) - t0, hi = std::min(toff[T + 1], t0 + cnt) - t0;
+                    Mask lanes{};
+                    for (int64_t i = lo; i < hi; i++) lanes[i / 64] |= 1ull << (i % 64);
+                    for (const auto &r : runs[T]) {
+                        ev.push_back({r.first, lanes});
+                        ev.push_back({r.second, lanes});
+                    }
                 }
-                if (S < s0 || S >= s1) continue;
-                Mask inc;
-                bool any = false;
-                for (int w = 0; w < kMaxTPL; w++) {
-                    if (w < tpl)
-                        inc[w] = include ? (sel[w] | ~act[w]) : ~sel[w];
-                    else
-                        inc[w] = ~0ull;
-                    if (w < tpl && (inc[w] & act[w])) any = true;
+                std::sort(ev.begin(), ev.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+                Mask cur{};
+                for (size_t k = 0; k < ev.size();) {
+                    const int pos = ev[k].first;
+                    while (k < ev.size() && ev[k].first == pos) {
+                        for (int w = 0; w < kMaxTPL; w++) cur[w] ^= ev[k].second[w];
+                        k++;
+                    }
+                    if (k < ev.size()) {
+                        bool any = false;
+                        for (int w = 0; w < kMaxTPL; w++) any = any || cur[w] != 0;
+                        if (any) {
+                            Mask m = cur;
+                            bool all = true;
+                            for (int w = 0; w < tpl; w++) all = all && (m[w] & act[w]) == act[w];
+                            emit(soff[pos], soff[ev[k].first], all ? full : m);
+                        }
+                    }
                 }
-                if (!include) emit(soff[cur], soff[S], full);
-                if (any) emit(soff[S], soff[S + 1], inc);
-                cur = S + 1;
             }
-            if (!include) emit(soff[cur], soff[s1], full);
+            const bool single = Ta == Tb;
+            last_T = single ? Ta : -1;
+            last_work = gwork;
+            last_slots.assign(split, {0, 0, 0});
             size_t iv_i = 0;
             int64_t iv_pos = giv.empty() ? 0 : giv[0].begin;
             for (int sl = 0; sl < split; sl++) {
@@ -425,6 +541,7 @@ void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sp
                 g.iv_end = (int64_t)ivs.size();
                 groups.push_back(g);
                 work.push_back(quota);
+                last_slots[sl] = {g.iv_begin, g.iv_end, quota};
                 if (aligned) {
                     const int T = tsph[t0];
                     for (int c = 0; c < 3; c++) cen.push_back(sph_pos[3 * T + c]);
@@ -454,6 +571,8 @@ void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sp
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return work[a] > work[b]; });
     L.ngroups = (int)groups.size();
     L.pairs = pairs;
+    if (setup_verbose())
+        std::fprintf(stderr, "[sphstk]     %d work items, %zu intervals, %.4g pairs\n", L.ngroups, ivs.size(), pairs);
     L.groups.upload(groups);
     L.iv.upload(ivs);
     L.order.upload(ord);
@@ -511,6 +630,7 @@ uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
 
 void Operator::ensure_levels(LevelSet &S) {
     if (S.ready) return;
+    PhaseTimer pt(S.fp64 ? "FP64 outer level set" : "FP32 inner level set");
     const double *a = S.alo;
     auto lorder = [&](int l, int p) { return S.order[l] < 0 ? level_order(l, p) : std::min(p, S.order[l]); };
     // Morton order of the spheres within each suspension
@@ -537,52 +657,49 @@ void Operator::ensure_levels(LevelSet &S) {
             rank[key[k].second] = s0 + k;
         }
     }
-    // neighbour classes: mid (fine), shells of levels 0..L-2, exclusion list of the last level
-    const double amax = a[kNumLevels - 1];
-    std::vector<std::vector<int>> mid(nsph_);
-    std::vector<std::vector<std::vector<int>>> shell(kNumLevels, std::vector<std::vector<int>>(nsph_));
-    for (int b = 0; b < nsusp_; b++) {
-        const int s0 = (int)susp_sph_off_[b], s1 = (int)susp_sph_off_[b + 1];
-        if (s1 <= s0) continue;
-        double rmax = 0, lo[3] = {1e300, 1e300, 1e300};
-        for (int i = s0; i < s1; i++) {
-            rmax = std::max(rmax, rad_[i]);
-            for (int c = 0; c < 3; c++) lo[c] = std::min(lo[c], pos_[3 * i + c]);
+    // Pair classes by one scan per target sphere over its suspension's spheres
+    // in Morton order: near (< 1.5 (R_s + R_t), the operator's near-pair
+    // predicate) and self are skipped, mid (< a0) is summed on the fine grid,
+    // level l covers [a_l, a_l+1).  Level runs are in Morton rank space, the
+    // mid runs in sphere index space.
+    auto tn = std::make_unique<PhaseTimer>("  level pair classes");
+    std::vector<Runs> mid(nsph_);
+    std::vector<std::vector<Runs>> lvl(kNumLevels, std::vector<Runs>(nsph_));
+    std::vector<int> sph_s0(nsph_), sph_s1(nsph_);
+    for (int b = 0; b < nsusp_; b++)
+        for (int64_t t = susp_sph_off_[b]; t < susp_sph_off_[b + 1]; t++) {
+            sph_s0[t] = (int)susp_sph_off_[b];
+            sph_s1[t] = (int)susp_sph_off_[b + 1];
         }
-        const double h = 2.0 * amax * rmax;
-        std::unordered_map<int64_t, std::vector<int>> cells;
-        auto key = [&](int64_t a, int64_t bb, int64_t c) { return (a * 1000003LL + bb) * 1000033LL + c; };
-        std::vector<std::array<int64_t, 3>> cid(s1 - s0);
-        for (int i = s0; i < s1; i++) {
-            for (int c = 0; c < 3; c++) cid[i - s0][c] = (int64_t)std::floor((pos_[3 * i + c] - lo[c]) / h);
-            cells[key(cid[i - s0][0], cid[i - s0][1], cid[i - s0][2])].push_back(i);
+    parallel_for(nsph_, [&](int ta, int tb) {
+        std::vector<int> ml;
+        for (int t = ta; t < tb; t++) {
+            const int s0 = sph_s0[t], s1 = sph_s1[t], rt = rank[t];
+            ml.clear();
+            for (int r = s0; r < s1; r++) {
+                const int s = perm[r];
+                if (s == t) continue;
+                const double ddx = pos_[3 * s] - pos_[3 * t], ddy = pos_[3 * s + 1] - pos_[3 * t + 1],
+                             ddz = pos_[3 * s + 2] - pos_[3 * t + 2];
+                const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                const double rr = 1.5 * rad_[s] + 1.5 * rad_[t], rs = rad_[s] + rad_[t];
+                if (d2 < rr * rr) continue;
+                if (d2 < a[0] * a[0] * rs * rs) {
+                    ml.push_back(s);
+                    continue;
+                }
+                const int l = d2 < a[1] * a[1] * rs * rs ? 0 : d2 < a[2] * a[2] * rs * rs ? 1 : 2;
+                run_append(lvl[l][rt], r);
+            }
+            std::sort(ml.begin(), ml.end());
+            mid[t] = runs_of(ml, s0, s1, true);
         }
-        for (int t = s0; t < s1; t++) {
-            const auto &ct = cid[t - s0];
-            for (int dx = -1; dx <= 1; dx++)
-                for (int dy = -1; dy <= 1; dy++)
-                    for (int dz = -1; dz <= 1; dz++) {
-                        auto it = cells.find(key(ct[0] + dx, ct[1] + dy, ct[2] + dz));
-                        if (it == cells.end()) continue;
-                        for (int s : it->second) {
-                            const double ddx = pos_[3 * s] - pos_[3 * t], ddy = pos_[3 * s + 1] - pos_[3 * t + 1],
-                                         ddz = pos_[3 * s + 2] - pos_[3 * t + 2];
-                            const double sep = std::sqrt(ddx * ddx + ddy * ddy + ddz * ddz) / (rad_[s] + rad_[t]);
-                            if (s == t || sep < amax) shell[kNumLevels - 1][rank[t]].push_back(rank[s]);
-                            if (s == t || sep < 1.5) continue;
-                            if (sep < a[0]) {
-                                mid[t].push_back(s);
-                                continue;
-                            }
-                            for (int l = 0; l + 1 < kNumLevels; l++)
-                                if (sep >= a[l] && sep < a[l + 1]) shell[l][rank[t]].push_back(rank[s]);
-                        }
-                    }
-            std::sort(mid[t].begin(), mid[t].end());
-            for (int l = 0; l < kNumLevels; l++) std::sort(shell[l][rank[t]].begin(), shell[l][rank[t]].end());
-        }
+    });
+    tn.reset();
+    {
+        PhaseTimer pm("  mid-range lists");
+        build_far_lists(S.mid, nsusp_, susp_sph_off_, pt_off_, pt_off_, mid, tpl_, 0, kWantItems);
     }
-    build_far_lists(S.mid, nsusp_, susp_sph_off_, pt_off_, pt_off_, mid, true, tpl_, 0, kWantItems);
     S.levels.clear();
     for (int l = 0; l < kNumLevels; l++) {
         auto L = std::make_unique<FarLevel>();
@@ -630,8 +747,8 @@ void Operator::ensure_levels(LevelSet &S) {
                 L->up.nin[p] = rs.nc;
                 L->up.nout[p] = rs.n;
             }
-        const bool last = (l == kNumLevels - 1);
-        build_far_lists(L->lists, nsusp_, susp_sph_off_, off, off, shell[l], !last, tpl_, 0, kWantItems);
+        PhaseTimer pl("  level lists");
+        build_far_lists(L->lists, nsusp_, susp_sph_off_, off, off, lvl[l], tpl_, 0, kWantItems);
         std::vector<int> dummy;
         L->chunks.upload(make_chunks(nsusp_, susp_sph_off_, off, &dummy));
         L->sv.alloc(std::max<int64_t>(3 * nc, 1));
@@ -862,9 +979,9 @@ std::vector<std::vector<int>> Operator::neighbours_within(double amax) const {
 // quadrature of order p reproduces to ~(R_t / d)^(2p+2), below 1e-10 for p >= 6
 // beyond 4 (R_s + R_t) when R_t / (R_s + R_t) <= 0.75).
 constexpr double kFaxenSep = 4.0;
-void Operator::rigid_velocity(const double *rho, double inv_mu, double *vel) {
-    if (nsph_ == 0) return;
+void Operator::ensure_centre() {
     if (!centre_ready_) {
+        PhaseTimer pt("final-velocity lists");
         std::vector<int64_t> toff(nsph_ + 1);
         for (int i = 0; i <= nsph_; i++) toff[i] = i;
         // excluded from the centre sums: the sphere itself and every sphere
@@ -879,16 +996,37 @@ void Operator::rigid_velocity(const double *rho, double inv_mu, double *vel) {
             excl[t].push_back(t);
             std::sort(excl[t].begin(), excl[t].end());
         }
-        build_far_lists(lists_rv_mid_, nsusp_, susp_sph_off_, pt_off_, pt_off_, grid_mid, true, tpl_, 0, kWantItems);
+        std::vector<Runs> rmid(nsph_), rcen(nsph_);
+        for (int t = 0; t < nsph_; t++) {
+            const int b = sph_susp_[t], s0 = (int)susp_sph_off_[b], s1 = (int)susp_sph_off_[b + 1];
+            rmid[t] = runs_of(grid_mid[t], s0, s1, true);
+            rcen[t] = runs_of(excl[t], s0, s1, false);
+        }
+        build_far_lists(lists_rv_mid_, nsusp_, susp_sph_off_, pt_off_, pt_off_, rmid, tpl_, 0, kWantItems);
         int64_t nb = 0;
         for (int b = 0; b < nsusp_; b++) nb += (susp_sph_off_[b + 1] - susp_sph_off_[b] + 63) / 64;
         const int split = (int)std::max<int64_t>(1, std::min<int64_t>(64, (kWantItems + nb - 1) / std::max<int64_t>(nb, 1)));
-        build_far_lists(lists_centre_, nsusp_, susp_sph_off_, toff, pt_off_, excl, false, 1, split, kWantItems);
+        build_far_lists(lists_centre_, nsusp_, susp_sph_off_, toff, pt_off_, rcen, 1, split, kWantItems);
         centre_part_.alloc(size_t(lists_centre_.split) * 9 * nsph_);
         vel_ns_.alloc(6 * (size_t)nsph_);
         u_ns_.alloc(std::max<int64_t>(3 * npts_, 1));
         centre_ready_ = true;
     }
+}
+
+// Builds everything the mixed-precision solve uses (otherwise built on first
+// use): the FP32 and FP64 level sets and the final-velocity lists.
+void Operator::prepare_mixed(bool outer_levels) {
+    if (nsph_ == 0) return;
+    if (multi_level_) ensure_levels(ls32_);
+    if (outer_levels) outer_levels_pay();
+    ensure_centre();
+    SPHSTK_HIP(hipStreamSynchronize(stream_));
+}
+
+void Operator::rigid_velocity(const double *rho, double inv_mu, double *vel) {
+    if (nsph_ == 0) return;
+    ensure_centre();
     launch_analysis(geom_, rho, coef_.p, nullptr, stream_, max_order_);
     launch_expand(geom_, 0, coef_.p, rec_.p, nullptr, stream_, max_order_);
     launch_fill(3 * npts_, 0.0, u_ns_.p, stream_);
@@ -993,6 +1131,7 @@ Mobility::Mobility(int nsusp, const int *counts, const double *pos, const double
     vel_.alloc(std::max(6 * A_->nsph(), 1));
     act_.alloc(A_->nsusp());
     scal_.alloc(4 * (size_t)A_->nsusp());
+    if (prec_ == 1) A_->prepare_mixed(outer_prec_ == 2);
     SPHSTK_HIP(hipStreamSynchronize(A_->stream()));
 }
 

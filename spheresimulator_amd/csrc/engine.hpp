@@ -49,10 +49,19 @@ struct FarLists {
     int64_t ntrg = 0;
 };
 
+// Half-open ranges [a, b) of sphere indices, sorted and disjoint.
+using Runs = std::vector<std::pair<int32_t, int32_t>>;
+inline void run_append(Runs &r, int s) {
+    if (!r.empty() && r.back().second == s)
+        r.back().second = s + 1;
+    else
+        r.push_back({s, s + 1});
+}
+Runs runs_of(const std::vector<int> &lst, int s0, int s1, bool include);
+
 void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sph_off,
-                     const std::vector<int64_t> &toff, const std::vector<int64_t> &soff,
-                     const std::vector<std::vector<int>> &lists, bool include, int tpl, int split,
-                     int64_t want_items, const double *sph_pos = nullptr);
+                     const std::vector<int64_t> &toff, const std::vector<int64_t> &soff, const std::vector<Runs> &runs,
+                     int tpl, int split, int64_t want_items, const double *sph_pos = nullptr);
 
 // One level of the multi-level FP32 far field: per-sphere grids of a lower
 // order, their geometry, resampling tables and work lists.
@@ -131,6 +140,8 @@ class Operator {
     void project(double *inout);
     // rigid-body velocities (6 per sphere) of the single layer of rho, distant pairs by Faxen's laws
     void rigid_velocity(const double *rho, double inv_mu, double *vel);
+    // builds the level sets and final-velocity lists of the mixed-precision solve now
+    void prepare_mixed(bool outer_levels);
 
     // host accessors
     void grid(double *weights, double *coords, double *norms, int64_t *index, int64_t *length) const;
@@ -155,6 +166,7 @@ class Operator {
     void ensure_levels(LevelSet &S);
     std::vector<std::vector<int>> neighbours_within(double amax) const;
     bool outer_levels_pay();
+    void ensure_centre();
     // multi-level far field of `in` into out; ev_a and ev_b (if set) bracket
     // the pair-sum launches
     void far_levels(LevelSet &S, int kind, double coef, const double *in, double *out, const int *active,
@@ -177,7 +189,8 @@ class Operator {
     std::vector<int32_t> pt_sph_;
 
     DevBuf<double> d_pos_, d_rad_, d_quat_, d_coords_, d_norms_, d_w_;
-    DevBuf<int> d_order_, d_sph_susp_, d_near_src_;
+    DevBuf<int> d_order_, d_sph_susp_, d_near_src_, d_tile_sph_;
+    DevBuf<SphTile> d_tiles_;
     DevBuf<int64_t> d_pt_off_, d_coef_off_, d_rec_off_, d_near_off_;
     DevBuf<double> d_anal_[kMaxOrder + 1];
     DevGeom geom_;

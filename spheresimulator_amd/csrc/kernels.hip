@@ -364,93 +364,101 @@ void launch_far(int kind, int prec, int tpl, const void *src, const double *tcoo
 }
 
 // ---------------------------------------------------------------------------
-// SH analysis: coef[S] = A_p * (q_S^-1 in_S), A_p column-major (6Q x 3n).
+// Per-sphere matrices applied to body-frame values, one tile of up to kTileSph
+// spheres of the same order per workgroup so that each matrix element is read
+// once per tile (the matrices are up to 12.7 MB at p = 16):
+//   analysis:  coef[S] = A_p (q_S^-1 in_S)                (A_p: 6Q x 3n)
+//   resample:  out_S (+)= q_S M_p (q_S^-1 in_S)            (M_p: 3 nout x 3 nin)
+// Each thread owns one matrix row (RB = 255 rows, i.e. 85 points, per
+// workgroup: blockIdx.y) and accumulates it for every sphere of the tile; the
+// rotated input values are staged through LDS in chunks of kChunkPts points,
+// stored [column][sphere] so the inner loop reads them as broadcasts.
+constexpr int kRowBlock = 255;
+constexpr int kChunkPts = 64;
 
-__global__ __launch_bounds__(256) void k_analysis(DevGeom g, const double *__restrict__ in,
-                                                  double *__restrict__ coef, const int *__restrict__ active) {
-    extern __shared__ double vb[];
-    const int S = blockIdx.x;
-    if (active && !active[g.sph_susp[S]]) return;
-    const int p = g.order[S];
-    const int n = (p + 1) * (2 * p + 2);
-    const int rows = 3 * (p + 1) * (p + 2);
-    const int cols = 3 * n;
-    const int64_t off = g.pt_off[S];
-    const double *q = g.quat + 4 * S;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const double v[3] = {in[3 * (off + i)], in[3 * (off + i) + 1], in[3 * (off + i) + 2]};
-        double o[3];
-        qrot(q, v, o, true);
-        vb[3 * i] = o[0];
-        vb[3 * i + 1] = o[1];
-        vb[3 * i + 2] = o[2];
-    }
-    __syncthreads();
-    const double *__restrict__ A = g.anal[p];
-    double *cS = coef + g.coef_off[S];
-    for (int r = threadIdx.x; r < rows; r += blockDim.x) {
-        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-        int c = 0;
-        for (; c + 3 < cols; c += 4) {
-            a0 = __builtin_fma(A[(int64_t)c * rows + r], vb[c], a0);
-            a1 = __builtin_fma(A[(int64_t)(c + 1) * rows + r], vb[c + 1], a1);
-            a2 = __builtin_fma(A[(int64_t)(c + 2) * rows + r], vb[c + 2], a2);
-            a3 = __builtin_fma(A[(int64_t)(c + 3) * rows + r], vb[c + 3], a3);
-        }
-        for (; c < cols; c++) a0 = __builtin_fma(A[(int64_t)c * rows + r], vb[c], a0);
-        cS[r] = (a0 + a1) + (a2 + a3);
-    }
-}
+struct MatTabs {
+    const double *M[kMaxOrder + 1] = {};
+    int rows[kMaxOrder + 1] = {};
+    int nin[kMaxOrder + 1] = {};
+};
 
-// Per-sphere resampling between two point sets of the sphere (fine grid and
-// coarse grid): out_S (+)= R_S M_p R_S^-1 in_S with M_p column-major
-// (3 nout x 3 nin) acting on body-frame values.
-__global__ __launch_bounds__(256) void k_resample(DevGeom g, ResampleTabs tab, const int64_t *__restrict__ in_off,
-                                                  const int64_t *__restrict__ out_off, const double *__restrict__ in,
-                                                  double *__restrict__ out, int accumulate,
-                                                  const int *__restrict__ active) {
-    extern __shared__ double lds_rs[];
-    const int S = blockIdx.x;
-    if (active && !active[g.sph_susp[S]]) return;
-    const int p = g.order[S];
+template <bool RESAMPLE>
+__global__ __launch_bounds__(256) void k_tiled_apply(DevGeom g, MatTabs tab, const int64_t *__restrict__ in_off,
+                                                     const int64_t *__restrict__ out_off,
+                                                     const double *__restrict__ in, double *__restrict__ out,
+                                                     int accumulate, const int *__restrict__ active) {
+    __shared__ double V[3 * kChunkPts * kTileSph];
+    __shared__ double O[RESAMPLE ? kRowBlock * kTileSph : 1];
+    const SphTile T = g.tiles[blockIdx.x];
+    const int p = T.p, rows = tab.rows[p], nin = tab.nin[p];
+    const int r0 = blockIdx.y * kRowBlock;
+    if (r0 >= rows) return;
     const double *__restrict__ M = tab.M[p];
-    const int nin = tab.nin[p], nout = tab.nout[p];
-    const int64_t io = in_off[S], oo = out_off[S];
-    const double *q = g.quat + 4 * S;
-    double *vin = lds_rs, *vout = lds_rs + 3 * nin;
-    for (int i = threadIdx.x; i < nin; i += blockDim.x) {
-        const double v[3] = {in[3 * (io + i)], in[3 * (io + i) + 1], in[3 * (io + i) + 2]};
-        double o[3];
-        qrot(q, v, o, true);
-        vin[3 * i] = o[0];
-        vin[3 * i + 1] = o[1];
-        vin[3 * i + 2] = o[2];
-    }
-    __syncthreads();
-    const int rows = 3 * nout, cols = 3 * nin;
-    for (int r = threadIdx.x; r < rows; r += blockDim.x) {
-        double a0 = 0, a1 = 0;
-        int c = 0;
-        for (; c + 1 < cols; c += 2) {
-            a0 = __builtin_fma(M[(int64_t)c * rows + r], vin[c], a0);
-            a1 = __builtin_fma(M[(int64_t)(c + 1) * rows + r], vin[c + 1], a1);
+    const int *sph = g.tile_sph + T.first;
+    const int tid = threadIdx.x;
+    const int r = r0 + tid;
+    const bool own = tid < kRowBlock && r < rows;
+    double acc[kTileSph];
+#pragma unroll
+    for (int t = 0; t < kTileSph; t++) acc[t] = 0.0;
+    for (int k0 = 0; k0 < nin; k0 += kChunkPts) {
+        const int kc = min(kChunkPts, nin - k0);
+        __syncthreads();
+        for (int e = tid; e < kChunkPts * kTileSph; e += blockDim.x) {
+            const int t = e / kChunkPts, i = e % kChunkPts;
+            double o[3] = {0.0, 0.0, 0.0};
+            if (t < T.count && i < kc) {
+                const int S = sph[t];
+                const double *v = in + 3 * (in_off[S] + k0 + i);
+                const double vv[3] = {v[0], v[1], v[2]};
+                qrot(g.quat + 4 * S, vv, o, true);
+            }
+            V[(3 * i) * kTileSph + t] = o[0];
+            V[(3 * i + 1) * kTileSph + t] = o[1];
+            V[(3 * i + 2) * kTileSph + t] = o[2];
         }
-        if (c < cols) a0 = __builtin_fma(M[(int64_t)c * rows + r], vin[c], a0);
-        vout[r] = a0 + a1;
+        __syncthreads();
+        if (own) {
+            const double *Mc = M + (int64_t)(3 * k0) * rows + r;
+            for (int c = 0; c < 3 * kc; c++) {
+                const double a = Mc[(int64_t)c * rows];
+                const double *vc = V + c * kTileSph;
+#pragma unroll
+                for (int t = 0; t < kTileSph; t++) acc[t] = __builtin_fma(a, vc[t], acc[t]);
+            }
+        }
     }
+    if (!RESAMPLE) {
+        if (!own) return;
+        for (int t = 0; t < T.count; t++) {
+            const int S = sph[t];
+            if (active && !active[g.sph_susp[S]]) continue;
+            out[g.coef_off[S] + r] = acc[t];
+        }
+        return;
+    }
+    // resample: rotate each output point back to the lab frame
+    if (own)
+#pragma unroll
+        for (int t = 0; t < kTileSph; t++) O[tid * kTileSph + t] = acc[t];
     __syncthreads();
-    for (int i = threadIdx.x; i < nout; i += blockDim.x) {
+    const int npt = (min(rows, r0 + kRowBlock) - r0) / 3;
+    for (int e = tid; e < npt * T.count; e += blockDim.x) {
+        const int t = e / npt, i = e % npt;
+        const int S = sph[t];
+        if (active && !active[g.sph_susp[S]]) continue;
+        const double vb[3] = {O[(3 * i) * kTileSph + t], O[(3 * i + 1) * kTileSph + t], O[(3 * i + 2) * kTileSph + t]};
         double o[3];
-        qrot(q, vout + 3 * i, o, false);
-        double *d = out + 3 * (oo + i);
+        qrot(g.quat + 4 * S, vb, o, false);
+        double *dst = out + 3 * (out_off[S] + r0 / 3 + i);
         if (accumulate) {
-            d[0] += o[0];
-            d[1] += o[1];
-            d[2] += o[2];
+            dst[0] += o[0];
+            dst[1] += o[1];
+            dst[2] += o[2];
         } else {
-            d[0] = o[0];
-            d[1] = o[1];
-            d[2] = o[2];
+            dst[0] = o[0];
+            dst[1] = o[1];
+            dst[2] = o[2];
         }
     }
 }
@@ -458,16 +466,27 @@ __global__ __launch_bounds__(256) void k_resample(DevGeom g, ResampleTabs tab, c
 void launch_resample(const DevGeom &g, const ResampleTabs &tab, const int64_t *in_off, const int64_t *out_off,
                      const double *in, double *out, int accumulate, const int *active, hipStream_t s,
                      int max_points) {
-    if (!g.nsph) return;
-    const size_t lds = sizeof(double) * 3 * 2 * (size_t)max_points;
-    k_resample<<<dim3(g.nsph), dim3(256), lds, s>>>(g, tab, in_off, out_off, in, out, accumulate, active);
+    if (!g.nsph || !g.ntiles) return;
+    MatTabs mt;
+    for (int p = 0; p <= kMaxOrder; p++) {
+        mt.M[p] = tab.M[p];
+        mt.rows[p] = 3 * tab.nout[p];
+        mt.nin[p] = tab.nin[p];
+    }
+    const int rb = (3 * max_points + kRowBlock - 1) / kRowBlock;
+    k_tiled_apply<true><<<dim3(g.ntiles, rb), dim3(256), 0, s>>>(g, mt, in_off, out_off, in, out, accumulate, active);
 }
-
 void launch_analysis(const DevGeom &g, const double *in, double *coef, const int *active, hipStream_t s,
                      int max_order) {
-    if (!g.nsph) return;
-    const size_t lds = sizeof(double) * 3 * (max_order + 1) * (2 * max_order + 2);
-    k_analysis<<<dim3(g.nsph), dim3(256), lds, s>>>(g, in, coef, active);
+    if (!g.nsph || !g.ntiles) return;
+    MatTabs mt;
+    for (int p = 0; p <= kMaxOrder; p++) {
+        mt.M[p] = g.anal[p];
+        mt.rows[p] = 3 * (p + 1) * (p + 2);
+        mt.nin[p] = (p + 1) * (2 * p + 2);
+    }
+    const int rb = (3 * (max_order + 1) * (max_order + 2) + kRowBlock - 1) / kRowBlock;
+    k_tiled_apply<false><<<dim3(g.ntiles, rb), dim3(256), 0, s>>>(g, mt, g.pt_off, nullptr, in, coef, 0, active);
 }
 
 // ---------------------------------------------------------------------------

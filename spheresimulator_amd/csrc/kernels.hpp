@@ -8,8 +8,18 @@ namespace sphstk {
 constexpr int kMaxOrder = 32;
 
 // Read-only device view of one (possibly batched) operator's geometry.
+// Spheres of one order, at most kTileSph of them (indices tile_sph[first ..
+// first + count)), processed together by the tiled per-sphere matrix kernels.
+constexpr int kTileSph = 16;
+struct SphTile {
+    int32_t p, first, count, pad;
+};
+
 struct DevGeom {
     int nsph = 0, nsusp = 0;
+    int ntiles = 0;
+    const SphTile *tiles = nullptr; // spheres grouped by order
+    const int *tile_sph = nullptr;
     int64_t npts = 0;
     const double *pos = nullptr;   // 3N
     const double *rad = nullptr;   // N
