@@ -50,7 +50,8 @@ extern "C" {
 /* creation flags */
 #define SPHSTK_FLAG_NONE 0
 #define SPHSTK_FLAG_TIMING 1      /* record HIP-event timings of the far-field kernels */
-#define SPHSTK_FLAG_FP32_DIRECT 2 /* FP32 inner operator: direct far field (no two-level coarse grids) */
+#define SPHSTK_FLAG_FP32_DIRECT 2 /* FP32 inner operator: direct far field (no multi-level coarse grids) */
+#define SPHSTK_FLAG_AUTOTUNE 4    /* time candidate far-field tile shapes at set-up and keep the fastest */
 
 typedef struct sphstk_operator sphstk_operator;
 typedef struct sphstk_mobility sphstk_mobility;
@@ -145,6 +146,12 @@ int sphstk_operator_get_grid(const sphstk_operator *op, double *weights, double 
 
 /* Near pairs (target, source) in the order used internally (grouped by target). */
 int sphstk_operator_get_near_pairs(const sphstk_operator *op, int32_t *trg, int32_t *src);
+
+/* Far-field work-list shapes in use (chosen by SPHSTK_FLAG_AUTOTUNE, else the
+ * defaults): targets per lane and source slices of the FP64 direct lists, and
+ * targets per lane of the FP32 inner operator's lists.  No counterpart in the
+ * reference. */
+int sphstk_operator_far_shape(const sphstk_operator *op, int *tpl, int *split, int *inner_tpl);
 
 /* Far-field kernel timing collected with SPHSTK_FLAG_TIMING: total milliseconds,
  * launches, and the algorithmic pair count per launch, for the traction (kind 0)

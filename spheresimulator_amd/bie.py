@@ -19,6 +19,7 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 FLAG_TIMING = 1
 FLAG_FP32_DIRECT = 2
+FLAG_AUTOTUNE = 4
 
 
 def _d(a):
@@ -48,7 +49,7 @@ class SphereSTKSHOperator:
     """Y := beta Y + alpha (cId I + cSL S + cTrac K' + cLOP L) X on the GPU."""
 
     def __init__(self, pos, radius, quat, order, cId=0.0, cSL=0.0, cTrac=0.0, cLOP=0.0, counts=None, timing=False,
-                 fp32_direct=False, _handle=None, _owner=None):
+                 fp32_direct=False, autotune=False, _handle=None, _owner=None):
         L = load()
         self._owner = _owner
         if _handle is not None:
@@ -62,7 +63,8 @@ class SphereSTKSHOperator:
                                                    s.order.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                                    cId, cSL, cTrac, cLOP,
                                                    (FLAG_TIMING if timing else 0) |
-                                                   (FLAG_FP32_DIRECT if fp32_direct else 0), ctypes.byref(h)))
+                                                   (FLAG_FP32_DIRECT if fp32_direct else 0) |
+                                                   (FLAG_AUTOTUNE if autotune else 0), ctypes.byref(h)))
             self._h = h
             self._own = True
         ns, npt, nd, nn = (ctypes.c_int64() for _ in range(4))
@@ -172,6 +174,12 @@ class SphereSTKSHOperator:
     def reset_stats(self):
         check(load().sphstk_operator_reset_stats(self._h))
 
+    def far_shape(self):
+        """Far-field work-list shapes in use: FP64 targets per lane and source slices, FP32 targets per lane."""
+        t, s, i = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(load().sphstk_operator_far_shape(self._h, ctypes.byref(t), ctypes.byref(s), ctypes.byref(i)))
+        return dict(tpl=t.value, split=s.value, inner_tpl=i.value)
+
 
 def solver_params(tol=1e-8, restart=30, max_iters=1000, explicit_check=1, inner_tol=1e-5, max_refine=10):
     return SolverParams(tol, restart, max_iters, explicit_check, inner_tol, max_refine)
@@ -181,7 +189,7 @@ class SphereSTKMobMat:
     """Force/torque (6N) -> velocity/angular velocity (6N) through A = 0.5 I + K' + L."""
 
     def __init__(self, pos, radius, quat, order, viscosity=1.0, counts=None, precision=PREC_FP64, params=None,
-                 timing=False, fp32_direct=False):
+                 timing=False, fp32_direct=False, autotune=False):
         L = load()
         s = _Spheres(pos, radius, quat, order, counts)
         self.n_spheres = s.n
@@ -192,7 +200,8 @@ class SphereSTKMobMat:
                                                s.order.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), viscosity,
                                                ctypes.byref(self.params), precision,
                                                (FLAG_TIMING if timing else 0) |
-                                               (FLAG_FP32_DIRECT if fp32_direct else 0), ctypes.byref(h)))
+                                               (FLAG_FP32_DIRECT if fp32_direct else 0) |
+                                               (FLAG_AUTOTUNE if autotune else 0), ctypes.byref(h)))
         self._h = h
         self.op = SphereSTKSHOperator(None, None, None, None, _handle=L.sphstk_mobility_operator(h), _owner=self)
 

@@ -266,6 +266,15 @@ int sphstk_operator_get_near_pairs(const sphstk_operator *op, int32_t *trg, int3
     });
 }
 
+int sphstk_operator_far_shape(const sphstk_operator *op, int *tpl, int *split, int *inner_tpl) {
+    return guard([&] {
+        SPHSTK_REQUIRE(op && op->op, "invalid operator handle");
+        if (tpl) *tpl = op->op->far_tpl();
+        if (split) *split = op->op->far_split();
+        if (inner_tpl) *inner_tpl = op->op->inner_tpl();
+    });
+}
+
 int sphstk_operator_kernel_stats(const sphstk_operator *op, int kind, int prec, double *total_ms, int64_t *launches,
                                  double *pairs_per_launch) {
     return guard([&] {

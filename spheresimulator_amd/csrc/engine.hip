@@ -23,6 +23,7 @@ namespace {
 constexpr int kMaxSupportedOrder = 20; // LDS of the expansion kernel bounds the order
 constexpr int64_t kChunk = 16384;
 constexpr int64_t kWantItems = 40000; // far-field work items wanted when a problem is small
+constexpr double kTuneBudget = 3.0;    // seconds per autotuning pass (SPHSTK_FLAG_AUTOTUNE)
 
 // SPHSTK_VERBOSE=1: set-up phase times on stderr
 static bool setup_verbose() {
@@ -70,6 +71,7 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     c_[1] = cSL;
     c_[2] = cTrac;
     c_[3] = cLOP;
+    SPHSTK_REQUIRE((flags & ~7) == 0, "unknown flags (known: TIMING 1, FP32_DIRECT 2, AUTOTUNE 4)");
     SPHSTK_REQUIRE(nsusp >= 1, "number of suspensions must be >= 1");
     SPHSTK_REQUIRE(counts != nullptr, "sphere counts must not be NULL");
     susp_sph_off_.assign(nsusp + 1, 0);
@@ -279,7 +281,6 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     int tpl = 1, split = 0;
     if (const char *v = std::getenv("SPHSTK_FAR_TPL")) tpl = std::atoi(v);
     if (const char *v = std::getenv("SPHSTK_FAR_SPLIT")) split = std::atoi(v);
-    tpl_ = tpl;
     // FP32 inner operator: orders from level_order(); FP64 outer applies: fine
     // below 8 (R_s + R_t), then orders 9, 7, 6 from 8, 16, 32 (R_s + R_t).
     ls32_.alo[0] = 4.0, ls32_.alo[1] = 8.0, ls32_.alo[2] = 14.0;
@@ -314,19 +315,157 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
             for (int64_t k = near_off_[t]; k < near_off_[t + 1]; k++) excl[t].push_back(near_src_[k]);
             std::sort(excl[t].begin(), excl[t].end());
         }
-        std::vector<Runs> runs(nsph_);
+        runs_fine_.assign(nsph_, Runs{});
         for (int t = 0; t < nsph_; t++) {
             const int b = sph_susp_[t];
-            runs[t] = runs_of(excl[t], (int)susp_sph_off_[b], (int)susp_sph_off_[b + 1], false);
+            runs_fine_[t] = runs_of(excl[t], (int)susp_sph_off_[b], (int)susp_sph_off_[b + 1], false);
         }
-        build_far_lists(lists_fine_, nsusp_, susp_sph_off_, pt_off_, pt_off_, runs, tpl, split, kWantItems,
+        build_far_lists(lists_fine_, nsusp_, susp_sph_off_, pt_off_, pt_off_, runs_fine_, tpl, split, kWantItems,
                         far_mfma_ ? pos_.data() : nullptr);
     }
+    ls32_.tpl = ls64_.tpl = tpl;
 
     src64_.alloc(std::max<int64_t>(8 * npts_, 8));
     coef_.alloc(std::max<int64_t>(coef_off_[nsph_], 1));
     rec_.alloc(std::max<int64_t>(rec_off_[nsph_], 1));
     tmp_.alloc(std::max<int64_t>(3 * npts_, 1));
+    if ((flags_ & 4) && std::getenv("SPHSTK_FAR_TPL") == nullptr && std::getenv("SPHSTK_FAR_SPLIT") == nullptr) {
+        double budget = kTuneBudget;
+        autotune_fine(budget);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tile-shape autotuner (SPHSTK_FLAG_AUTOTUNE).  The far-field kernels take two
+// shape parameters: targets per lane (work items of 64, 128 or 256 targets)
+// and the number of source slices per item (more, smaller items balance
+// better; each slice adds a partial-sum pass).  Candidates are timed on this
+// problem's own work lists with a unit density, two launches each (the first
+// warms caches and code), within a time budget so that very large problems
+// keep the default shapes instead of spending minutes tuning.
+double Operator::time_far_pass(FarLists &L, int prec, const void *src, const double *tcoords, const double *tnorms,
+                               const float *trg32, const Chunk *chunks, int nchunks, double *out) {
+    hipEvent_t a, b;
+    SPHSTK_HIP(hipEventCreate(&a));
+    SPHSTK_HIP(hipEventCreate(&b));
+    far_run(L, 0, prec, src, tcoords, tnorms, trg32, chunks, nchunks, 1.0, out, nullptr);
+    SPHSTK_HIP(hipEventRecord(a, stream_));
+    far_run(L, 0, prec, src, tcoords, tnorms, trg32, chunks, nchunks, 1.0, out, nullptr);
+    SPHSTK_HIP(hipEventRecord(b, stream_));
+    SPHSTK_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    SPHSTK_HIP(hipEventElapsedTime(&ms, a, b));
+    SPHSTK_HIP(hipEventDestroy(a));
+    SPHSTK_HIP(hipEventDestroy(b));
+    return ms;
+}
+
+void Operator::autotune_fine(double &budget_s) {
+    if (nsph_ == 0 || far_mfma_) return;
+    launch_fill(3 * npts_, 1.0, tmp_.p, stream_);
+    launch_pack_src_f64(npts_, d_coords_.p, d_w_.p, tmp_.p, src64_.p, stream_);
+    DevBuf<double> out;
+    out.alloc(std::max<int64_t>(3 * npts_, 1));
+    auto timed = [&](FarLists &L) {
+        const double ms = time_far_pass(L, 0, src64_.p, d_coords_.p, d_norms_.p, nullptr, chunks_.p,
+                                        (int)chunks_.n, out.p);
+        budget_s -= 2e-3 * ms;
+        return ms;
+    };
+    double best = timed(lists_fine_);
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "fine FP64: tpl %d split %d %.3f ms", lists_fine_.tpl, lists_fine_.split, best);
+    tune_log_ += buf;
+    const int s0 = lists_fine_.split;
+    for (int tpl : {1, 2, 4})
+        for (int f : {1, 2, 4}) {
+            const int split = std::max(1, std::min(64, f == 1 ? s0 : f == 2 ? s0 / 2 : 2 * s0));
+            if (tpl == lists_fine_.tpl && split == lists_fine_.split) continue;
+            if (budget_s < 2e-3 * best * 1.5) break;
+            FarLists C;
+            build_far_lists(C, nsusp_, susp_sph_off_, pt_off_, pt_off_, runs_fine_, tpl, split, kWantItems);
+            const double ms = timed(C);
+            std::snprintf(buf, sizeof buf, "; tpl %d split %d %.3f ms", tpl, split, ms);
+            tune_log_ += buf;
+            if (ms < best) {
+                best = ms;
+                lists_fine_ = std::move(C);
+            }
+        }
+    ls64_.tpl = lists_fine_.tpl;
+    std::snprintf(buf, sizeof buf, " -> tpl %d split %d\n", lists_fine_.tpl, lists_fine_.split);
+    tune_log_ += buf;
+    if (setup_verbose()) std::fprintf(stderr, "[sphstk] autotune %s", tune_log_.c_str());
+}
+
+// FP32 inner operator: targets per lane of all its level lists, timed as one
+// multi-level pass.
+void Operator::far_levels_pass(LevelSet &S, const double *in, double *out) {
+    launch_pack_src_f32(npts_, d_coords_.p, d_w_.p, in, src32_.p, stream_);
+    far_levels(S, 0, 1.0, in, out, nullptr, nullptr, nullptr);
+}
+
+void Operator::autotune_inner(double &budget_s) {
+    if (nsph_ == 0 || !multi_level_) return;
+    if (!trg32_.p) {
+        trg32_.alloc(8 * std::max<int64_t>(npts_, 1));
+        src32_.alloc(8 * std::max<int64_t>(npts_, 1));
+        launch_pack_trg_f32(npts_, d_coords_.p, d_norms_.p, trg32_.p, stream_);
+    }
+    launch_fill(3 * npts_, 1.0, tmp_.p, stream_);
+    DevBuf<double> out;
+    out.alloc(std::max<int64_t>(3 * npts_, 1));
+    auto timed = [&]() {
+        auto t0 = std::chrono::steady_clock::now();
+        far_levels_pass(ls32_, tmp_.p, out.p);
+        hipEvent_t a, b;
+        SPHSTK_HIP(hipEventCreate(&a));
+        SPHSTK_HIP(hipEventCreate(&b));
+        SPHSTK_HIP(hipEventRecord(a, stream_));
+        far_levels_pass(ls32_, tmp_.p, out.p);
+        SPHSTK_HIP(hipEventRecord(b, stream_));
+        SPHSTK_HIP(hipEventSynchronize(b));
+        float ms = 0.f;
+        SPHSTK_HIP(hipEventElapsedTime(&ms, a, b));
+        SPHSTK_HIP(hipEventDestroy(a));
+        SPHSTK_HIP(hipEventDestroy(b));
+        budget_s -= std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return (double)ms;
+    };
+    ensure_levels(ls32_);
+    const int t_def = ls32_.tpl;
+    double best = timed();
+    int best_tpl = t_def;
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "FP32 levels: tpl %d %.3f ms", t_def, best);
+    std::string log = buf;
+    for (int tpl : {1, 2, 4}) {
+        if (tpl == t_def) continue;
+        if (budget_s < 0) break;
+        LevelSet keep = std::move(ls32_);
+        ls32_.tpl = tpl;
+        std::copy(keep.alo, keep.alo + kNumLevels, ls32_.alo);
+        std::copy(keep.order, keep.order + kNumLevels, ls32_.order);
+        ls32_.fp64 = false;
+        ls32_.ready = false;
+        ls32_.levels.clear();
+        auto tb = std::chrono::steady_clock::now();
+        ensure_levels(ls32_);
+        budget_s -= std::chrono::duration<double>(std::chrono::steady_clock::now() - tb).count();
+        const double ms = timed();
+        std::snprintf(buf, sizeof buf, "; tpl %d %.3f ms", tpl, ms);
+        log += buf;
+        if (ms < best) {
+            best = ms;
+            best_tpl = tpl;
+        } else {
+            ls32_ = std::move(keep);
+        }
+    }
+    std::snprintf(buf, sizeof buf, " -> tpl %d\n", best_tpl);
+    log += buf;
+    tune_log_ += log;
+    if (setup_verbose()) std::fprintf(stderr, "[sphstk] autotune %s", log.c_str());
 }
 
 Operator::~Operator() {
@@ -698,7 +837,7 @@ void Operator::ensure_levels(LevelSet &S) {
     tn.reset();
     {
         PhaseTimer pm("  mid-range lists");
-        build_far_lists(S.mid, nsusp_, susp_sph_off_, pt_off_, pt_off_, mid, tpl_, 0, kWantItems);
+        build_far_lists(S.mid, nsusp_, susp_sph_off_, pt_off_, pt_off_, mid, S.tpl, 0, kWantItems);
     }
     S.levels.clear();
     for (int l = 0; l < kNumLevels; l++) {
@@ -748,7 +887,7 @@ void Operator::ensure_levels(LevelSet &S) {
                 L->up.nout[p] = rs.n;
             }
         PhaseTimer pl("  level lists");
-        build_far_lists(L->lists, nsusp_, susp_sph_off_, off, off, lvl[l], tpl_, 0, kWantItems);
+        build_far_lists(L->lists, nsusp_, susp_sph_off_, off, off, lvl[l], S.tpl, 0, kWantItems);
         std::vector<int> dummy;
         L->chunks.upload(make_chunks(nsusp_, susp_sph_off_, off, &dummy));
         L->sv.alloc(std::max<int64_t>(3 * nc, 1));
@@ -1002,7 +1141,7 @@ void Operator::ensure_centre() {
             rmid[t] = runs_of(grid_mid[t], s0, s1, true);
             rcen[t] = runs_of(excl[t], s0, s1, false);
         }
-        build_far_lists(lists_rv_mid_, nsusp_, susp_sph_off_, pt_off_, pt_off_, rmid, tpl_, 0, kWantItems);
+        build_far_lists(lists_rv_mid_, nsusp_, susp_sph_off_, pt_off_, pt_off_, rmid, lists_fine_.tpl, 0, kWantItems);
         int64_t nb = 0;
         for (int b = 0; b < nsusp_; b++) nb += (susp_sph_off_[b + 1] - susp_sph_off_[b] + 63) / 64;
         const int split = (int)std::max<int64_t>(1, std::min<int64_t>(64, (kWantItems + nb - 1) / std::max<int64_t>(nb, 1)));
@@ -1018,7 +1157,13 @@ void Operator::ensure_centre() {
 // use): the FP32 and FP64 level sets and the final-velocity lists.
 void Operator::prepare_mixed(bool outer_levels) {
     if (nsph_ == 0) return;
-    if (multi_level_) ensure_levels(ls32_);
+    if (multi_level_) {
+        if ((flags_ & 4) && std::getenv("SPHSTK_FAR_TPL") == nullptr) {
+            double budget = kTuneBudget;
+            autotune_inner(budget);
+        }
+        ensure_levels(ls32_);
+    }
     if (outer_levels) outer_levels_pay();
     ensure_centre();
     SPHSTK_HIP(hipStreamSynchronize(stream_));
@@ -1160,6 +1305,7 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
     const int64_t n = A.ndof();
     if (V_.n < size_t(m + 1) * n) V_.alloc(size_t(m + 1) * n);
     h_.alloc(size_t(2 * (m + 1) + 1) * B);
+    if (prec == 1 && m + 1 <= 64) mdot_.alloc(size_t(m + 1) * std::max(A.nchunks(), 1));
     coefv_.alloc(size_t(m) * B);
     const Chunk *ch = A.chunks();
     const int nch = A.nchunks();
@@ -1219,13 +1365,27 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
             const double *Vj = V_.p + size_t(j) * n;
             A.apply_into(Vj, W, prec, act_.p);
             applies++;
-            for (int pass = 0; pass < 2; pass++)
+            // h = V^T w with two orthogonalisation passes.  FP64 solve: IMGS, the
+            // reference's Belos setting (mobilitySolver.yaml:3-16), one basis
+            // vector at a time.  Inner FP32-operator solve: CGS2 (classical
+            // Gram-Schmidt twice: all j+1 products of a pass in one kernel),
+            // which stores -h.
+            const bool cgs2 = prec == 1 && m + 1 <= 64;
+            for (int pass = 0; pass < 2; pass++) {
+                if (cgs2) {
+                    double *hv = h_.p + size_t(pass * (m + 1)) * B;
+                    launch_seg_multi_dot(W, V_.p, n, j + 1, ch, nch, mdot_.p, act_.p, st);
+                    launch_seg_multi_reduce(mdot_.p, nch, sco, B, j + 1, hv, 2, act_.p, st);
+                    launch_seg_multi_axpy(W, V_.p, n, hv, j + 1, B, ch, nch, act_.p, st);
+                    continue;
+                }
                 for (int i = 0; i <= j; i++) {
                     double *hv = h_.p + size_t(pass * (m + 1) + i) * B;
                     launch_seg_dot(W, V_.p + size_t(i) * n, ch, nch, part, act_.p, st);
                     launch_seg_reduce(part, sco, B, hv, 0, act_.p, st);
                     launch_seg_axpy(W, V_.p + size_t(i) * n, hv, -1.0, ch, nch, act_.p, st);
                 }
+            }
             double *nv = h_.p + size_t(2 * (m + 1)) * B;
             launch_seg_dot(W, W, ch, nch, part, act_.p, st);
             launch_seg_reduce(part, sco, B, nv, 0, act_.p, st);
@@ -1238,8 +1398,9 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
                 double *Hs = H.data() + size_t(s) * (m + 1) * m;
                 double *cs_ = cs.data() + size_t(s) * m, *sn_ = sn.data() + size_t(s) * m;
                 double *gs = g.data() + size_t(s) * (m + 1);
+                const double hsign = cgs2 ? -1.0 : 1.0;
                 for (int i = 0; i <= j; i++)
-                    Hs[i * m + j] = hh[size_t(i) * B + s] + hh[size_t((m + 1) + i) * B + s];
+                    Hs[i * m + j] = hsign * (hh[size_t(i) * B + s] + hh[size_t((m + 1) + i) * B + s]);
                 const double hn = std::sqrt(std::max(hh[size_t(2 * (m + 1)) * B + s], 0.0));
                 Hs[(j + 1) * m + j] = hn;
                 for (int i = 0; i < j; i++) {

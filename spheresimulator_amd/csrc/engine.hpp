@@ -7,6 +7,7 @@
 #include "kernels.hpp"
 
 #include <memory>
+#include <string>
 
 namespace sphstk {
 
@@ -18,6 +19,20 @@ struct DevBuf {
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
     ~DevBuf() { release(); }
     void release() {
         if (p) (void)hipFree(p);
@@ -85,6 +100,7 @@ struct LevelSet {
     double alo[kNumLevels];  // separations (in units of R_s + R_t) where levels 0, 1, 2 start
     int order[kNumLevels];   // grid order of each level (capped by the sphere's order)
     bool fp64 = false;       // FP64 pair sums (else FP32)
+    int tpl = 1;             // targets per lane of its work lists
     std::vector<std::unique_ptr<FarLevel>> levels;
     FarLists mid;
     bool ready = false;
@@ -148,7 +164,9 @@ class Operator {
     void near_pairs(int32_t *trg, int32_t *src) const;
     void kernel_stats(int kind, int prec, double *ms, int64_t *launches, double *pairs);
     void reset_stats();
-    int far_tpl() const { return tpl_; }
+    int far_tpl() const { return lists_fine_.tpl; }
+    int inner_tpl() const { return ls32_.tpl; }
+    const std::string &tune_log() const { return tune_log_; }
     int far_split() const { return lists_fine_.split; }
 
     // segmented-vector helpers (one segment per suspension)
@@ -164,6 +182,12 @@ class Operator {
     void far_run(FarLists &L, int kind, int prec, const void *src, const double *tcoords, const double *tnorms,
                  const float *trg32, const Chunk *chunks, int nchunks, double coef, double *out, const int *active);
     void ensure_levels(LevelSet &S);
+    // tile-shape autotuner: times candidate work-list shapes on this problem
+    void autotune_fine(double &budget_s);
+    void autotune_inner(double &budget_s);
+    double time_far_pass(FarLists &L, int prec, const void *src, const double *tcoords, const double *tnorms,
+                         const float *trg32, const Chunk *chunks, int nchunks, double *out);
+    void far_levels_pass(LevelSet &S, const double *in, double *out);
     std::vector<std::vector<int>> neighbours_within(double amax) const;
     bool outer_levels_pay();
     void ensure_centre();
@@ -176,7 +200,6 @@ class Operator {
     int64_t npts_;
     double c_[4];
     int flags_;
-    int tpl_ = 1;
     int max_order_ = 0;
     hipStream_t stream_ = nullptr;
     int device_ = 0;
@@ -198,6 +221,8 @@ class Operator {
     // far-field work lists: fine grid (FP64 and direct FP32); the multi-level
     // far fields of the FP32 inner operator and of the FP64 outer applies
     FarLists lists_fine_;
+    std::vector<Runs> runs_fine_; // source runs of the direct far field per target sphere
+    std::string tune_log_;         // autotuner decisions
     LevelSet ls32_, ls64_;
     bool multi_level_ = true, timing_done_ = false;
     int outer_pay_ = -1; // FP64 multi-level outer far field in use: -1 undecided, 0 no, 1 yes
@@ -260,7 +285,7 @@ class Mobility {
     int outer_prec_ = 2; // far field of the mixed form's FP64 applies: 2 multi-level, 0 direct
     bool x_zero_ = true; // the stored unknown is known to be zero (cold start)
     bool verbose_ = false; // SPHSTK_VERBOSE=1: residual history on stderr (Belos Verbosity 63)
-    DevBuf<double> rho_, b_, x_, r_, d_, V_, h_, scal_, coefv_, force_, vel_, gmres_inner_rhs_;
+    DevBuf<double> rho_, b_, x_, r_, d_, V_, h_, scal_, coefv_, force_, vel_, gmres_inner_rhs_, mdot_;
     DevBuf<int> act_;
 };
 

@@ -943,6 +943,51 @@ __global__ __launch_bounds__(256) void k_seg_multi_axpy(double *__restrict__ x, 
     }
 }
 
+// Block-partial dot products of w with k basis vectors V_j (stride ldv) over
+// one chunk: partial[j * nchunks + chunk], j < k <= kMaxDots, in one pass over w.
+constexpr int kMaxDots = 64;
+__global__ __launch_bounds__(256) void k_seg_multi_dot(const double *__restrict__ w, const double *__restrict__ V,
+                                                       int64_t ldv, int k, const Chunk *__restrict__ ch,
+                                                       double *__restrict__ partial, const int *__restrict__ active) {
+    __shared__ double sh[4][kMaxDots];
+    const Chunk c = ch[blockIdx.x];
+    const bool on = !active || active[c.seg];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int j0 = 0; j0 < k; j0 += 16) {
+        double s[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) s[u] = 0.0;
+        if (on)
+            for (int64_t i = c.begin + threadIdx.x; i < c.end; i += blockDim.x) {
+                const double wi = w[i];
+#pragma unroll
+                for (int u = 0; u < 16; u++)
+                    if (j0 + u < k) s[u] = __builtin_fma(wi, V[(int64_t)(j0 + u) * ldv + i], s[u]);
+            }
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            double v = s[u];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (lane == 0) sh[wid][j0 + u < kMaxDots ? j0 + u : 0] = v;
+        }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < k; j += blockDim.x)
+        partial[(int64_t)j * gridDim.x + blockIdx.x] = (sh[0][j] + sh[1][j]) + (sh[2][j] + sh[3][j]);
+}
+
+// out[j * nseg + seg] = (op 2: -) sum of the chunk partials of seg, for j < k
+__global__ void k_seg_multi_reduce(const double *__restrict__ partial, int nchunks, const int *__restrict__ off,
+                                   int nseg, int k, double *__restrict__ out, int op, const int *__restrict__ active) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nseg * k) return;
+    const int sgi = idx % nseg, j = idx / nseg;
+    if (active && !active[sgi]) return;
+    double s = 0.0;
+    for (int q = off[sgi]; q < off[sgi + 1]; q++) s += partial[(int64_t)j * nchunks + q];
+    out[(int64_t)j * nseg + sgi] = op == 2 ? -s : s;
+}
+
 // out[i] += sum_k part[k*stride + i], k = 0..nslot-1, over the DOF chunks of active segments
 __global__ __launch_bounds__(256) void k_slot_reduce(double *__restrict__ out, const double *__restrict__ part,
                                                      int nslot, int64_t stride, const Chunk *__restrict__ ch,
@@ -992,6 +1037,18 @@ void launch_seg_multi_axpy(double *x, const double *V, int64_t ldv, const double
                            const Chunk *chunks, int nchunks, const int *active, hipStream_t s) {
     if (nchunks > 0 && k > 0)
         k_seg_multi_axpy<<<dim3(nchunks), dim3(256), 0, s>>>(x, V, ldv, coef, k, nseg, chunks, active);
+}
+void launch_seg_multi_dot(const double *w, const double *V, int64_t ldv, int k, const Chunk *chunks, int nchunks,
+                          double *partial, const int *active, hipStream_t s) {
+    if (k > kMaxDots) throw Error(-1, "internal: too many dot products in one pass");
+    if (nchunks > 0 && k > 0)
+        k_seg_multi_dot<<<dim3(nchunks), dim3(256), 0, s>>>(w, V, ldv, k, chunks, partial, active);
+}
+void launch_seg_multi_reduce(const double *partial, int nchunks, const int *off, int nseg, int k, double *out, int op,
+                             const int *active, hipStream_t s) {
+    const int n = nseg * k;
+    if (n > 0)
+        k_seg_multi_reduce<<<dim3((n + 63) / 64), dim3(64), 0, s>>>(partial, nchunks, off, nseg, k, out, op, active);
 }
 void launch_axpby(int64_t n, double a, const double *x, double b, double *y, hipStream_t s) {
     if (n > 0) k_axpby<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(n, a, x, b, y);

@@ -112,6 +112,12 @@ void launch_seg_axpy(double *y, const double *x, const double *alpha, double sig
 void launch_seg_scale(double *y, const double *x, const double *alpha, const Chunk *chunks, int nchunks,
                       const int *active, hipStream_t s);
 // x += sum_i coef[i*nseg + seg] * V[i]   (i < k)
+// partial[j * nchunks + chunk] = w . V_j over the chunk, j < k <= 64 (one pass over w)
+void launch_seg_multi_dot(const double *w, const double *V, int64_t ldv, int k, const Chunk *chunks, int nchunks,
+                          double *partial, const int *active, hipStream_t s);
+// out[j * nseg + seg] = sum of seg's chunk partials (op 2: negated), j < k
+void launch_seg_multi_reduce(const double *partial, int nchunks, const int *off, int nseg, int k, double *out, int op,
+                             const int *active, hipStream_t s);
 void launch_seg_multi_axpy(double *x, const double *V, int64_t ldv, const double *coef, int k, int nseg,
                            const Chunk *chunks, int nchunks, const int *active, hipStream_t s);
 // out += sum of nslot partial-sum slots (far-field source slices)

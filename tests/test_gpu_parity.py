@@ -279,3 +279,24 @@ def test_mixed_matches_fp64_low_order_polydisperse():
     ev, ew = rel(a[:, :3], b[:, :3]), rel(a[:, 3:], b[:, 3:])
     print("rel err V %.3e Omega %.3e" % (ev, ew))
     assert ev < 1e-8 and ew < 1e-8
+
+
+def test_autotuned_shapes_give_the_same_operator():
+    """SPHSTK_FLAG_AUTOTUNE picks far-field tile shapes by timing candidates on
+    the problem itself; whatever it picks, the operators are unchanged (up to
+    summation order)."""
+    say("autotuned far-field shapes")
+    c = S.configs.make_suspension(256, 8, 35, vf=0.05, min_gap=0.2)
+    args = (c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+    ref = S.SphereSTKSHOperator(*args)
+    tuned = S.SphereSTKSHOperator(*args, autotune=True)
+    sh = tuned.far_shape()
+    print("shape", sh, "default", ref.far_shape())
+    assert sh["tpl"] in (1, 2, 4) and 1 <= sh["split"] <= 64
+    x = np.random.default_rng(13).standard_normal(ref.n_dof)
+    assert rel(tuned.apply(x), ref.apply(x)) < 1e-13
+    M = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], precision=S.PREC_MIXED, autotune=True)
+    assert M.op.far_shape()["inner_tpl"] in (1, 2, 4)
+    v, info = M.solveMob(c["force"])
+    vr, _ = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"]).solveMob(c["force"])
+    assert info["converged"] == 1 and rel(v, vr) < 1e-8
