@@ -52,6 +52,7 @@ extern "C" {
 #define SPHSTK_FLAG_TIMING 1      /* record HIP-event timings of the far-field kernels */
 #define SPHSTK_FLAG_FP32_DIRECT 2 /* FP32 inner operator: direct far field (no multi-level coarse grids) */
 #define SPHSTK_FLAG_AUTOTUNE 4    /* time candidate far-field tile shapes at set-up and keep the fastest */
+#define SPHSTK_FLAG_GRAPH 8       /* replay the FP32 inner operator's apply from a captured HIP graph */
 
 typedef struct sphstk_operator sphstk_operator;
 typedef struct sphstk_mobility sphstk_mobility;

@@ -20,6 +20,7 @@ PREC_MIXED = 1
 FLAG_TIMING = 1
 FLAG_FP32_DIRECT = 2
 FLAG_AUTOTUNE = 4
+FLAG_GRAPH = 8
 
 
 def _d(a):
@@ -49,7 +50,7 @@ class SphereSTKSHOperator:
     """Y := beta Y + alpha (cId I + cSL S + cTrac K' + cLOP L) X on the GPU."""
 
     def __init__(self, pos, radius, quat, order, cId=0.0, cSL=0.0, cTrac=0.0, cLOP=0.0, counts=None, timing=False,
-                 fp32_direct=False, autotune=False, _handle=None, _owner=None):
+                 fp32_direct=False, autotune=False, graph=False, _handle=None, _owner=None):
         L = load()
         self._owner = _owner
         if _handle is not None:
@@ -64,7 +65,8 @@ class SphereSTKSHOperator:
                                                    cId, cSL, cTrac, cLOP,
                                                    (FLAG_TIMING if timing else 0) |
                                                    (FLAG_FP32_DIRECT if fp32_direct else 0) |
-                                                   (FLAG_AUTOTUNE if autotune else 0), ctypes.byref(h)))
+                                                   (FLAG_AUTOTUNE if autotune else 0) |
+                                                   (FLAG_GRAPH if graph else 0), ctypes.byref(h)))
             self._h = h
             self._own = True
         ns, npt, nd, nn = (ctypes.c_int64() for _ in range(4))
@@ -189,7 +191,7 @@ class SphereSTKMobMat:
     """Force/torque (6N) -> velocity/angular velocity (6N) through A = 0.5 I + K' + L."""
 
     def __init__(self, pos, radius, quat, order, viscosity=1.0, counts=None, precision=PREC_FP64, params=None,
-                 timing=False, fp32_direct=False, autotune=False):
+                 timing=False, fp32_direct=False, autotune=False, graph=False):
         L = load()
         s = _Spheres(pos, radius, quat, order, counts)
         self.n_spheres = s.n
@@ -201,7 +203,8 @@ class SphereSTKMobMat:
                                                ctypes.byref(self.params), precision,
                                                (FLAG_TIMING if timing else 0) |
                                                (FLAG_FP32_DIRECT if fp32_direct else 0) |
-                                               (FLAG_AUTOTUNE if autotune else 0), ctypes.byref(h)))
+                                               (FLAG_AUTOTUNE if autotune else 0) |
+                                               (FLAG_GRAPH if graph else 0), ctypes.byref(h)))
         self._h = h
         self.op = SphereSTKSHOperator(None, None, None, None, _handle=L.sphstk_mobility_operator(h), _owner=self)
 

@@ -71,7 +71,7 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     c_[1] = cSL;
     c_[2] = cTrac;
     c_[3] = cLOP;
-    SPHSTK_REQUIRE((flags & ~7) == 0, "unknown flags (known: TIMING 1, FP32_DIRECT 2, AUTOTUNE 4)");
+    SPHSTK_REQUIRE((flags & ~15) == 0, "unknown flags (known: TIMING 1, FP32_DIRECT 2, AUTOTUNE 4, GRAPH 8)");
     SPHSTK_REQUIRE(nsusp >= 1, "number of suspensions must be >= 1");
     SPHSTK_REQUIRE(counts != nullptr, "sphere counts must not be NULL");
     susp_sph_off_.assign(nsusp + 1, 0);
@@ -470,6 +470,8 @@ void Operator::autotune_inner(double &budget_s) {
 
 Operator::~Operator() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (gx_inner_) (void)hipGraphExecDestroy(gx_inner_);
+    if (g_inner_) (void)hipGraphDestroy(g_inner_);
     for (auto &e : pending_) {
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
@@ -985,7 +987,7 @@ bool Operator::outer_levels_pay() {
 // all far pairs.  FP32: multi-level (fine for mid-range pairs, coarser grids
 // with distance) unless disabled, in which case the FP32 direct sum.
 void Operator::far(int kind, int prec, double coef, const double *in, double *out, const int *active) {
-    const bool timing = (flags_ & 1) != 0;
+    const bool timing = (flags_ & 1) != 0 && !capturing_;
     EvPair ev{};
     if (timing) {
         if (free_events_.empty()) {
@@ -1197,9 +1199,57 @@ void Operator::apply_into(const double *in, double *out, int prec, const int *ac
     lop(in, out, c_[3], active);
 }
 
+// The inner operator's apply as one HIP graph: captured on the first call (or
+// when the `active` array changes), replayed afterwards between two copies to
+// and from fixed buffers.  The captured work is kernels only (the fills are
+// kernels, not memsets); timing events are not recorded while capturing.
+void Operator::apply_inner_into(const double *in, double *out, const int *active) {
+    if (!(flags_ & 8)) {
+        apply_into(in, out, 1, active);
+        return;
+    }
+    const int64_t n = 3 * npts_;
+    if (gx_inner_ && g_act_ == active) {
+        SPHSTK_HIP(hipMemcpyAsync(g_in_.p, in, sizeof(double) * n, hipMemcpyDeviceToDevice, stream_));
+        SPHSTK_HIP(hipGraphLaunch(gx_inner_, stream_));
+        SPHSTK_HIP(hipMemcpyAsync(out, g_out_.p, sizeof(double) * n, hipMemcpyDeviceToDevice, stream_));
+        return;
+    }
+    // this call runs eagerly, which also completes every lazy allocation and
+    // attribute setting before the capture
+    apply_into(in, out, 1, active);
+    if (gx_inner_) {
+        SPHSTK_HIP(hipGraphExecDestroy(gx_inner_));
+        SPHSTK_HIP(hipGraphDestroy(g_inner_));
+        gx_inner_ = nullptr;
+        g_inner_ = nullptr;
+    }
+    g_in_.alloc(std::max<int64_t>(n, 1));
+    g_out_.alloc(std::max<int64_t>(n, 1));
+    SPHSTK_HIP(hipStreamSynchronize(stream_));
+    SPHSTK_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    capturing_ = true;
+    try {
+        apply_into(g_in_.p, g_out_.p, 1, active);
+    } catch (...) {
+        capturing_ = false;
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(stream_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    capturing_ = false;
+    SPHSTK_HIP(hipStreamEndCapture(stream_, &g_inner_));
+    SPHSTK_HIP(hipGraphInstantiate(&gx_inner_, g_inner_, nullptr, nullptr, 0));
+    g_act_ = active;
+}
+
 // apply (cpp:153-236): Y := beta Y + alpha A X
 void Operator::apply(const double *X, double *Y, double alpha, double beta, int prec) {
-    apply_into(X, tmp_.p, prec, nullptr);
+    if (prec == 1)
+        apply_inner_into(X, tmp_.p, nullptr);
+    else
+        apply_into(X, tmp_.p, prec, nullptr);
     launch_axpby(3 * npts_, alpha, tmp_.p, beta, Y, stream_);
 }
 
@@ -1363,7 +1413,10 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
         for (int j = 0; j < m; j++) {
             double *W = V_.p + size_t(j + 1) * n;
             const double *Vj = V_.p + size_t(j) * n;
-            A.apply_into(Vj, W, prec, act_.p);
+            if (prec == 1)
+                A.apply_inner_into(Vj, W, act_.p);
+            else
+                A.apply_into(Vj, W, prec, act_.p);
             applies++;
             // h = V^T w with two orthogonalisation passes.  FP64 solve: IMGS, the
             // reference's Belos setting (mobilitySolver.yaml:3-16), one basis

@@ -153,6 +153,9 @@ class Operator {
     void apply_into(const double *in, double *out, int prec, const int *active);
     // Y = beta Y + alpha A X
     void apply(const double *X, double *Y, double alpha, double beta, int prec);
+    // out = A32 in (the FP32 inner operator); with SPHSTK_FLAG_GRAPH replayed
+    // from a HIP graph captured on the first call for this `active` array
+    void apply_inner_into(const double *in, double *out, const int *active);
     void project(double *inout);
     // rigid-body velocities (6 per sphere) of the single layer of rho, distant pairs by Faxen's laws
     void rigid_velocity(const double *rho, double inv_mu, double *vel);
@@ -227,6 +230,13 @@ class Operator {
     bool multi_level_ = true, timing_done_ = false;
     int outer_pay_ = -1; // FP64 multi-level outer far field in use: -1 undecided, 0 no, 1 yes
     bool far_mfma_ = false; // FP64 traction far field with the dot products on the matrix cores (opt-in)
+
+    // captured inner-operator apply (SPHSTK_FLAG_GRAPH): g_in_ -> g_out_
+    hipGraph_t g_inner_ = nullptr;
+    hipGraphExec_t gx_inner_ = nullptr;
+    const int *g_act_ = nullptr;
+    DevBuf<double> g_in_, g_out_;
+    bool capturing_ = false;
 
     // rigid_velocity: fine-grid sums of the pairs closer than kFaxenSep, sphere-centre sums of the rest
     FarLists lists_rv_mid_, lists_centre_;

@@ -300,3 +300,27 @@ def test_autotuned_shapes_give_the_same_operator():
     v, info = M.solveMob(c["force"])
     vr, _ = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"]).solveMob(c["force"])
     assert info["converged"] == 1 and rel(v, vr) < 1e-8
+
+
+def test_inner_apply_graph_replays_match_eager():
+    """SPHSTK_FLAG_GRAPH: the FP32 inner operator's apply is captured once and
+    replayed.  Every replay, with other eager work and synchronisations in
+    between, equals the eager apply of the same input; the mixed solve with
+    graphs equals the one without."""
+    say("inner apply HIP graph replays")
+    c = S.configs.make_suspension(128, 8, 37, vf=0.05, min_gap=0.2)
+    args = (c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+    eager = S.SphereSTKSHOperator(*args)
+    gr = S.SphereSTKSHOperator(*args, graph=True)
+    rng = np.random.default_rng(17)
+    for k in range(5):
+        x = rng.standard_normal(eager.n_dof)
+        want = eager.apply_inner(x)
+        got = gr.apply_inner(x)
+        print("replay %d rel diff %.3e" % (k, rel(got, want)))
+        assert rel(got, want) <= 1e-14
+        gr.apply(x)  # eager FP64 work between the replays
+    kw = dict(precision=S.PREC_MIXED)
+    v0, i0 = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], **kw).solveMob(c["force"])
+    v1, i1 = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], graph=True, **kw).solveMob(c["force"])
+    assert i1["converged"] == 1 and i1["iters"] == i0["iters"] and rel(v1, v0) <= 1e-13
