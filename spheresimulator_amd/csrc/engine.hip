@@ -283,22 +283,23 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     if (const char *v = std::getenv("SPHSTK_FAR_SPLIT")) split = std::atoi(v);
     // FP32 inner operator: orders from level_order(); FP64 outer applies: fine
     // below 8 (R_s + R_t), then orders 9, 7, 6 from 8, 16, 32 (R_s + R_t).
-    ls32_.alo[0] = 4.0, ls32_.alo[1] = 8.0, ls32_.alo[2] = 14.0;
-    ls32_.order[0] = ls32_.order[1] = ls32_.order[2] = -1;
+    ls32_.alo[0] = 4.0, ls32_.alo[1] = 8.0, ls32_.alo[2] = 14.0, ls32_.alo[3] = 28.0;
+    ls32_.order[0] = ls32_.order[1] = ls32_.order[2] = ls32_.order[3] = -1;
     ls32_.fp64 = false;
-    ls64_.alo[0] = 8.0, ls64_.alo[1] = 16.0, ls64_.alo[2] = 32.0;
-    ls64_.order[0] = 9, ls64_.order[1] = 7, ls64_.order[2] = 6;
+    ls64_.alo[0] = 8.0, ls64_.alo[1] = 16.0, ls64_.alo[2] = 32.0, ls64_.alo[3] = 64.0;
+    ls64_.order[0] = 9, ls64_.order[1] = 7, ls64_.order[2] = 6, ls64_.order[3] = 5;
     ls64_.fp64 = true;
     if (const char *v = std::getenv("SPHSTK_LEVEL_ALPHA"))
-        if (std::sscanf(v, "%lf,%lf,%lf", &ls32_.alo[0], &ls32_.alo[1], &ls32_.alo[2]) != 3)
-            throw Error(-1, "SPHSTK_LEVEL_ALPHA: a0,a1,a2");
+        if (std::sscanf(v, "%lf,%lf,%lf,%lf", &ls32_.alo[0], &ls32_.alo[1], &ls32_.alo[2], &ls32_.alo[3]) != 4)
+            throw Error(-1, "SPHSTK_LEVEL_ALPHA: a0,a1,a2,a3");
     if (const char *v = std::getenv("SPHSTK_OUTER_LEVELS"))
-        if (std::sscanf(v, "%lf,%lf,%lf:%d,%d,%d", &ls64_.alo[0], &ls64_.alo[1], &ls64_.alo[2], &ls64_.order[0],
-                        &ls64_.order[1], &ls64_.order[2]) != 6)
-            throw Error(-1, "SPHSTK_OUTER_LEVELS: a0,a1,a2:p0,p1,p2");
+        if (std::sscanf(v, "%lf,%lf,%lf,%lf:%d,%d,%d,%d", &ls64_.alo[0], &ls64_.alo[1], &ls64_.alo[2], &ls64_.alo[3],
+                        &ls64_.order[0], &ls64_.order[1], &ls64_.order[2], &ls64_.order[3]) != 8)
+            throw Error(-1, "SPHSTK_OUTER_LEVELS: a0,a1,a2,a3:p0,p1,p2,p3");
     for (const LevelSet *S : {&ls32_, &ls64_}) {
-        SPHSTK_REQUIRE(S->alo[0] >= 1.5 && S->alo[1] > S->alo[0] && S->alo[2] > S->alo[1],
-                       "level separations must increase from >= 1.5");
+        SPHSTK_REQUIRE(S->alo[0] >= 1.5, "level separations must start at >= 1.5");
+        for (int l = 1; l < kNumLevels; l++)
+            SPHSTK_REQUIRE(S->alo[l] > S->alo[l - 1], "level separations must increase");
         for (int l = 0; l < kNumLevels; l++)
             SPHSTK_REQUIRE(S->order[l] == -1 || (S->order[l] >= 1 && S->order[l] <= kMaxOrder),
                            "level orders must lie in [1, max order]");
@@ -753,6 +754,7 @@ int level_order(int lvl, int p) {
     if (lvl == 0) pc = p < 10 ? p : p / 2 + 1;
     if (lvl == 1) pc = std::max(5, p / 3 + 1);
     if (lvl == 2) pc = 4;
+    if (lvl == 3) pc = 3;
     return std::min(pc, p);
 }
 uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
@@ -829,7 +831,8 @@ void Operator::ensure_levels(LevelSet &S) {
                     ml.push_back(s);
                     continue;
                 }
-                const int l = d2 < a[1] * a[1] * rs * rs ? 0 : d2 < a[2] * a[2] * rs * rs ? 1 : 2;
+                int l = 0;
+                while (l + 1 < kNumLevels && d2 >= a[l + 1] * a[l + 1] * rs * rs) l++;
                 run_append(lvl[l][rt], r);
             }
             std::sort(ml.begin(), ml.end());
@@ -1310,6 +1313,9 @@ Mobility::Mobility(int nsusp, const int *counts, const double *pos, const double
                    const int *order, double viscosity, const SolverParams &prm, int precision, int flags)
     : mu_(viscosity), prm_(prm), prec_(precision) {
     if (const char *v = std::getenv("SPHSTK_OUTER_ML")) outer_prec_ = std::atoi(v) != 0 ? 2 : 0;
+    inner_restart_ = std::max(prm.restart, 60);
+    if (const char *v = std::getenv("SPHSTK_INNER_RESTART")) inner_restart_ = std::atoi(v);
+    SPHSTK_REQUIRE(inner_restart_ >= 1 && inner_restart_ <= 1000, "SPHSTK_INNER_RESTART must lie in [1, 1000]");
     SPHSTK_REQUIRE(std::isfinite(viscosity) && viscosity > 0, "viscosity must be > 0");
     SPHSTK_REQUIRE(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (mixed)");
     if (const char *v = std::getenv("SPHSTK_VERBOSE")) verbose_ = std::atoi(v) != 0;
@@ -1351,7 +1357,7 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
     Operator &A = *A_;
     hipStream_t st = A.stream();
     const int B = A.nsusp();
-    const int m = std::max(1, prm_.restart);
+    const int m = std::max(1, prec == 1 ? inner_restart_ : prm_.restart);
     const int64_t n = A.ndof();
     if (V_.n < size_t(m + 1) * n) V_.alloc(size_t(m + 1) * n);
     h_.alloc(size_t(2 * (m + 1) + 1) * B);

@@ -95,9 +95,9 @@ struct FarLevel {
 
 // A multi-level far field: fine-grid sums for the pairs closer than the first
 // level's separation (mid), then one coarser grid per separation shell.
-constexpr int kNumLevels = 3;
+constexpr int kNumLevels = 4;
 struct LevelSet {
-    double alo[kNumLevels];  // separations (in units of R_s + R_t) where levels 0, 1, 2 start
+    double alo[kNumLevels];  // separations (in units of R_s + R_t) where levels 0..3 start
     int order[kNumLevels];   // grid order of each level (capped by the sphere's order)
     bool fp64 = false;       // FP64 pair sums (else FP32)
     int tpl = 1;             // targets per lane of its work lists
@@ -293,6 +293,7 @@ class Mobility {
     SolverParams prm_;
     int prec_;
     int outer_prec_ = 2; // far field of the mixed form's FP64 applies: 2 multi-level, 0 direct
+    int inner_restart_ = 30; // GMRES restart of the inner FP32-operator solve
     bool x_zero_ = true; // the stored unknown is known to be zero (cold start)
     bool verbose_ = false; // SPHSTK_VERBOSE=1: residual history on stderr (Belos Verbosity 63)
     DevBuf<double> rho_, b_, x_, r_, d_, V_, h_, scal_, coefv_, force_, vel_, gmres_inner_rhs_, mdot_;
