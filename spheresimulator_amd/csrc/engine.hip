@@ -924,6 +924,7 @@ void Operator::harvest_events() {
 void Operator::far_run(FarLists &L, int kind, int prec, const void *src, const double *tcoords,
                        const double *tnorms, const float *trg32, const Chunk *chunks, int nchunks, double coef,
                        double *out, const int *active) {
+    if (L.pairs == 0.0) return; // e.g. a level that no sphere pair reaches
     const bool mfma = kind == 0 && prec == 0 && L.aligned && far_mfma_;
     double *dst = L.split > 1 ? L.part.p : out;
     const int acc = L.split > 1 ? 0 : 1;
@@ -932,7 +933,7 @@ void Operator::far_run(FarLists &L, int kind, int prec, const void *src, const d
                                  L.groups.p, L.iv.p, active, dst, coef, acc, stream_);
     else
         launch_far(kind, prec, L.tpl, src, tcoords, tnorms, trg32, L.ntrg, L.ngroups, L.order.p, L.groups.p, L.iv.p,
-                   active, dst, coef, acc, stream_);
+                   active, dst, coef, acc, stream_, &L != &lists_fine_);
     if (L.split > 1) launch_slot_reduce(out, L.part.p, L.split, 3 * L.ntrg, chunks, nchunks, active, stream_);
 }
 

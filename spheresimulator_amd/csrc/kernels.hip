@@ -222,12 +222,11 @@ __device__ __forceinline__ void store_far(const FarGroup &G, int lane, const dou
 }
 
 template <int KIND, int TPL>
-__global__ __launch_bounds__(64) void k_far_f64(const double *__restrict__ src, const double *__restrict__ coords,
-                                                const double *__restrict__ norms, const int *__restrict__ gorder,
-                                                const FarGroup *__restrict__ groups,
-                                                const FarInterval *__restrict__ iv, const int *__restrict__ active,
-                                                double *__restrict__ out, double coef, int accumulate,
-                                                int64_t slot_stride) {
+__device__ __forceinline__ void far_f64_body(const double *__restrict__ src, const double *__restrict__ coords,
+                                             const double *__restrict__ norms, const int *__restrict__ gorder,
+                                             const FarGroup *__restrict__ groups, const FarInterval *__restrict__ iv,
+                                             const int *__restrict__ active, double *__restrict__ out, double coef,
+                                             int accumulate, int64_t slot_stride) {
     const int gi = gorder[blockIdx.x];
     const FarGroup G = groups[gi];
     if (active && !active[G.susp]) return;
@@ -260,6 +259,28 @@ __global__ __launch_bounds__(64) void k_far_f64(const double *__restrict__ src, 
             far_range<KIND, TPL, true>(src, b, e, x, y, z, nx, ny, nz, incl, ax, ay, az);
     }
     store_far<TPL>(G, lane, ax, ay, az, out, KIND == 0 ? kTracW * coef : coef, accumulate, slot_stride);
+}
+
+// The direct FP64 far field (the whole fine grid) and the same sum over the
+// work lists of a multi-level far field, under two kernel names so that
+// profiles keep them apart.
+template <int KIND, int TPL>
+__global__ __launch_bounds__(64) void k_far_f64(const double *__restrict__ src, const double *__restrict__ coords,
+                                                const double *__restrict__ norms, const int *__restrict__ gorder,
+                                                const FarGroup *__restrict__ groups,
+                                                const FarInterval *__restrict__ iv, const int *__restrict__ active,
+                                                double *__restrict__ out, double coef, int accumulate,
+                                                int64_t slot_stride) {
+    far_f64_body<KIND, TPL>(src, coords, norms, gorder, groups, iv, active, out, coef, accumulate, slot_stride);
+}
+template <int KIND, int TPL>
+__global__ __launch_bounds__(64) void k_far_f64_lvl(const double *__restrict__ src, const double *__restrict__ coords,
+                                                    const double *__restrict__ norms, const int *__restrict__ gorder,
+                                                    const FarGroup *__restrict__ groups,
+                                                    const FarInterval *__restrict__ iv, const int *__restrict__ active,
+                                                    double *__restrict__ out, double coef, int accumulate,
+                                                    int64_t slot_stride) {
+    far_f64_body<KIND, TPL>(src, coords, norms, gorder, groups, iv, active, out, coef, accumulate, slot_stride);
 }
 
 // far field, FP32 pair arithmetic with two-level accumulation: FP32 partial sums
@@ -325,6 +346,7 @@ struct FarLaunch {
     const double *tcoords, *tnorms;
     const float *trg32;
     int64_t ntrg;
+    bool levels;
 };
 
 template <int KIND, int TPL>
@@ -332,7 +354,10 @@ static void far_dispatch_prec(int prec, const FarLaunch &a, int ngroups, const i
                               const FarInterval *iv, const int *active, double *out, double coef, int acc,
                               hipStream_t s) {
     const int64_t stride = 3 * a.ntrg;
-    if (prec == 0)
+    if (prec == 0 && a.levels)
+        k_far_f64_lvl<KIND, TPL><<<dim3(ngroups), dim3(64), 0, s>>>((const double *)a.src, a.tcoords, a.tnorms, go,
+                                                                   gr, iv, active, out, coef, acc, stride);
+    else if (prec == 0)
         k_far_f64<KIND, TPL><<<dim3(ngroups), dim3(64), 0, s>>>((const double *)a.src, a.tcoords, a.tnorms, go, gr,
                                                                iv, active, out, coef, acc, stride);
     else
@@ -354,9 +379,10 @@ static void far_dispatch_tpl(int tpl, int prec, const FarLaunch &a, int ngroups,
 
 void launch_far(int kind, int prec, int tpl, const void *src, const double *tcoords, const double *tnorms,
                 const float *trg32, int64_t ntrg, int ngroups, const int *go, const FarGroup *gr,
-                const FarInterval *iv, const int *active, double *out, double coef, int acc, hipStream_t s) {
+                const FarInterval *iv, const int *active, double *out, double coef, int acc, hipStream_t s,
+                bool levels) {
     if (ngroups <= 0) return;
-    const FarLaunch a{src, tcoords, tnorms, trg32, ntrg};
+    const FarLaunch a{src, tcoords, tnorms, trg32, ntrg, levels};
     if (kind == 0)
         far_dispatch_tpl<0>(tpl, prec, a, ngroups, go, gr, iv, active, out, coef, acc, s);
     else

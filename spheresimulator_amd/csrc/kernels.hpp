@@ -62,7 +62,8 @@ void launch_pack_trg_f32(int64_t n, const double *coords, const double *norms, f
 // accumulate=0: slot k of out (stride 3*ntrg) = coef * the sum over source slice k
 void launch_far(int kind, int prec, int tpl, const void *src, const double *tcoords, const double *tnorms,
                 const float *trg32, int64_t ntrg, int ngroups, const int *group_order, const FarGroup *groups,
-                const FarInterval *iv, const int *active, double *out, double coef, int accumulate, hipStream_t s);
+                const FarInterval *iv, const int *active, double *out, double coef, int accumulate, hipStream_t s,
+                bool levels = false);
 // FP64 traction far field on sphere-aligned lists with the dot products on the
 // matrix cores (far_mfma.hip); gcen holds each group's sphere centre.
 void launch_far_trac_mfma_f64(const double *src, const double *tcoords, const double *tnorms, const double *gcen,
