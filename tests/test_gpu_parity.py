@@ -205,8 +205,8 @@ def test_warm_start_and_density():
 def test_inner_fp32_operator_accuracy(p, direct, tol):
     """The FP32 inner operator approximates the FP64 operator on a white band-
     limited density (the projection of a random field).  Direct FP32 far field:
-    FP32 rounding only.  Two-level far field (pairs beyond 4 (R_s + R_t) through
-    coarse grids of order p/2+1): the truncated degrees decay like (1/8)^(p/2+2)."""
+    FP32 rounding only.  Multi-level far field (pairs beyond 4 (R_s + R_t) through
+    coarser grids, order p/2+1 first): the truncated degrees decay like (1/8)^(p/2+2)."""
     say("inner FP32 operator (p=%d direct=%s)" % (p, direct))
     c = S.configs.make_suspension(96 if p == 8 else 256, p, 31, vf=0.1, min_gap=0.2)
     gpu = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0,
