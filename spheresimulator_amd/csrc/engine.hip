@@ -21,7 +21,7 @@ namespace sphstk {
 namespace {
 
 constexpr int kMaxSupportedOrder = 20; // LDS of the expansion kernel bounds the order
-constexpr int64_t kChunk = 16384;
+constexpr int64_t kChunk = 4096;  // DOF per block of the segmented vector kernels
 constexpr int64_t kWantItems = 40000; // far-field work items wanted when a problem is small
 constexpr double kTuneBudget = 3.0;    // seconds per autotuning pass (SPHSTK_FLAG_AUTOTUNE)
 

@@ -395,11 +395,12 @@ void launch_far(int kind, int prec, int tpl, const void *src, const double *tcoo
 // once per tile (the matrices are up to 12.7 MB at p = 16):
 //   analysis:  coef[S] = A_p (q_S^-1 in_S)                (A_p: 6Q x 3n)
 //   resample:  out_S (+)= q_S M_p (q_S^-1 in_S)            (M_p: 3 nout x 3 nin)
-// Each thread owns one matrix row (RB = 255 rows, i.e. 85 points, per
-// workgroup: blockIdx.y) and accumulates it for every sphere of the tile; the
-// rotated input values are staged through LDS in chunks of kChunkPts points,
-// stored [column][sphere] so the inner loop reads them as broadcasts.
-constexpr int kRowBlock = 255;
+// A workgroup owns kRowBlock = 63 matrix rows (21 points) of one tile: lane l
+// of each of its four waves owns row r0 + l and the wave sums a quarter of
+// the columns of every chunk; the four partial sums meet in LDS at the end.
+// The rotated input values are staged through LDS in chunks of kChunkPts
+// points, stored [column][sphere] so the inner loop reads them as broadcasts.
+constexpr int kRowBlock = 63;
 constexpr int kChunkPts = 64;
 
 struct MatTabs {
@@ -414,16 +415,16 @@ __global__ __launch_bounds__(256) void k_tiled_apply(DevGeom g, MatTabs tab, con
                                                      const double *__restrict__ in, double *__restrict__ out,
                                                      int accumulate, const int *__restrict__ active) {
     __shared__ double V[3 * kChunkPts * kTileSph];
-    __shared__ double O[RESAMPLE ? kRowBlock * kTileSph : 1];
+    __shared__ double P[4][64 * kTileSph];
     const SphTile T = g.tiles[blockIdx.x];
     const int p = T.p, rows = tab.rows[p], nin = tab.nin[p];
     const int r0 = blockIdx.y * kRowBlock;
     if (r0 >= rows) return;
     const double *__restrict__ M = tab.M[p];
     const int *sph = g.tile_sph + T.first;
-    const int tid = threadIdx.x;
-    const int r = r0 + tid;
-    const bool own = tid < kRowBlock && r < rows;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int r = r0 + lane;
+    const bool own = lane < kRowBlock && r < rows;
     double acc[kTileSph];
 #pragma unroll
     for (int t = 0; t < kTileSph; t++) acc[t] = 0.0;
@@ -445,8 +446,22 @@ __global__ __launch_bounds__(256) void k_tiled_apply(DevGeom g, MatTabs tab, con
         }
         __syncthreads();
         if (own) {
+            // this wave's quarter of the chunk's columns, four matrix loads in flight
+            const int nc = 3 * kc, cq = (nc + 3) / 4, c0 = wv * cq, c1 = min(nc, c0 + cq);
             const double *Mc = M + (int64_t)(3 * k0) * rows + r;
-            for (int c = 0; c < 3 * kc; c++) {
+            int c = c0;
+            for (; c + 4 <= c1; c += 4) {
+                double a[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) a[u] = Mc[(int64_t)(c + u) * rows];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const double *vc = V + (c + u) * kTileSph;
+#pragma unroll
+                    for (int t = 0; t < kTileSph; t++) acc[t] = __builtin_fma(a[u], vc[t], acc[t]);
+                }
+            }
+            for (; c < c1; c++) {
                 const double a = Mc[(int64_t)c * rows];
                 const double *vc = V + c * kTileSph;
 #pragma unroll
@@ -454,26 +469,30 @@ __global__ __launch_bounds__(256) void k_tiled_apply(DevGeom g, MatTabs tab, con
             }
         }
     }
+    // the four waves' partial sums, added in a fixed order
+#pragma unroll
+    for (int t = 0; t < kTileSph; t++) P[wv][lane * kTileSph + t] = acc[t];
+    __syncthreads();
+    for (int e = tid; e < 64 * kTileSph; e += blockDim.x) P[0][e] = (P[0][e] + P[1][e]) + (P[2][e] + P[3][e]);
+    __syncthreads();
     if (!RESAMPLE) {
-        if (!own) return;
-        for (int t = 0; t < T.count; t++) {
+        const int nr = min(rows, r0 + kRowBlock) - r0;
+        for (int e = tid; e < nr * T.count; e += blockDim.x) {
+            const int t = e / nr, i = e % nr;
             const int S = sph[t];
             if (active && !active[g.sph_susp[S]]) continue;
-            out[g.coef_off[S] + r] = acc[t];
+            out[g.coef_off[S] + r0 + i] = P[0][i * kTileSph + t];
         }
         return;
     }
     // resample: rotate each output point back to the lab frame
-    if (own)
-#pragma unroll
-        for (int t = 0; t < kTileSph; t++) O[tid * kTileSph + t] = acc[t];
-    __syncthreads();
     const int npt = (min(rows, r0 + kRowBlock) - r0) / 3;
     for (int e = tid; e < npt * T.count; e += blockDim.x) {
         const int t = e / npt, i = e % npt;
         const int S = sph[t];
         if (active && !active[g.sph_susp[S]]) continue;
-        const double vb[3] = {O[(3 * i) * kTileSph + t], O[(3 * i + 1) * kTileSph + t], O[(3 * i + 2) * kTileSph + t]};
+        const double vb[3] = {P[0][(3 * i) * kTileSph + t], P[0][(3 * i + 1) * kTileSph + t],
+                              P[0][(3 * i + 2) * kTileSph + t]};
         double o[3];
         qrot(g.quat + 4 * S, vb, o, false);
         double *dst = out + 3 * (out_off[S] + r0 / 3 + i);
