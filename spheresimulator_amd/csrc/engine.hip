@@ -1152,6 +1152,7 @@ void Operator::ensure_centre() {
         for (int b = 0; b < nsusp_; b++) nb += (susp_sph_off_[b + 1] - susp_sph_off_[b] + 63) / 64;
         const int split = (int)std::max<int64_t>(1, std::min<int64_t>(64, (kWantItems + nb - 1) / std::max<int64_t>(nb, 1)));
         build_far_lists(lists_centre_, nsusp_, susp_sph_off_, toff, pt_off_, rcen, 1, split, kWantItems);
+        build_far_lists(lists_rigid_, nsusp_, susp_sph_off_, pt_off_, toff, rcen, 1, 1, kWantItems);
         centre_part_.alloc(size_t(lists_centre_.split) * 9 * nsph_);
         vel_ns_.alloc(6 * (size_t)nsph_);
         u_ns_.alloc(std::max<int64_t>(3 * npts_, 1));
@@ -1173,6 +1174,21 @@ void Operator::prepare_mixed(bool outer_levels) {
     if (outer_levels) outer_levels_pay();
     ensure_centre();
     SPHSTK_HIP(hipStreamSynchronize(stream_));
+}
+
+void Operator::rigid_p2p(const double *force, const double *rho0, double *out, double cId, double cTrac) {
+    if (nsph_ == 0) return;
+    ensure_centre();
+    launch_analysis(geom_, rho0, coef_.p, nullptr, stream_, max_order_);
+    launch_pack_src_f64(npts_, d_coords_.p, d_w_.p, rho0, src64_.p, stream_);
+    far_run(lists_rv_mid_, 0, 0, src64_.p, d_coords_.p, d_norms_.p, nullptr, chunks_.p, (int)chunks_.n,
+            -3.0 / (4.0 * kPi) * cTrac, out, nullptr);
+    launch_far_rigid(d_coords_.p, d_norms_.p, d_pos_.p, force, d_rad_.p, npts_, lists_rigid_.ngroups,
+                     lists_rigid_.order.p, lists_rigid_.groups.p, lists_rigid_.iv.p, out, cTrac, 1, stream_);
+    launch_expand(geom_, 1, coef_.p, rec_.p, nullptr, stream_, max_order_);
+    launch_sheval(geom_, 1, rec_.p, items_.p, (int)items_.n, d_trg_slot_off_.p, d_trg_nsrc_.p, evscratch_.p, rho0, out,
+                  0.0, cTrac, cId, nullptr, stream_);
+    SPHSTK_HIP(hipGetLastError());
 }
 
 void Operator::rigid_velocity(const double *rho, double inv_mu, double *vel) {
@@ -1559,7 +1575,10 @@ void Mobility::solve(const double *force, double *vel, std::vector<SegInfo> &inf
     // b = -(1/2 I + K') rho0   (:196)
     launch_fill(n, 0.0, b_.p, st);
     const int p64 = prec_ == 0 ? 0 : outer_prec_; // FP64 direct, or FP64 multi-level in the mixed form
-    A.p2p(rho_.p, b_.p, -0.5, 0.0, -1.0, p64, nullptr);
+    if (prec_ == 0)
+        A.p2p(rho_.p, b_.p, -0.5, 0.0, -1.0, 0, nullptr);
+    else // distant spheres' rho0 in closed form (exact to the grid quadrature's ~(R/d)^(2p+2))
+        A.rigid_p2p(force_.p, rho_.p, b_.p, -0.5, -1.0);
     app64++;
     std::vector<double> bn;
     seg_norms(b_.p, bn);

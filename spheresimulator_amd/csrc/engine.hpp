@@ -161,6 +161,10 @@ class Operator {
     void rigid_velocity(const double *rho, double inv_mu, double *vel);
     // builds the level sets and final-velocity lists of the mixed-precision solve now
     void prepare_mixed(bool outer_levels);
+    // out += cId rho0 + cTrac K'[rho0] for the rigid-body density rho0 of `force`
+    // (6 per sphere): self/near and pairs closer than 4 (R_s + R_t) as p2p, distant
+    // pairs in closed form at the sphere centres
+    void rigid_p2p(const double *force, const double *rho0, double *out, double cId, double cTrac);
 
     // host accessors
     void grid(double *weights, double *coords, double *norms, int64_t *index, int64_t *length) const;
@@ -240,6 +244,7 @@ class Operator {
 
     // rigid_velocity: fine-grid sums of the pairs closer than kFaxenSep, sphere-centre sums of the rest
     FarLists lists_rv_mid_, lists_centre_;
+    FarLists lists_rigid_; // grid targets x distant sphere centres (closed-form rigid-density traction)
     bool centre_ready_ = false;
     DevBuf<double> centre_part_, vel_ns_, u_ns_;
 

@@ -1179,6 +1179,80 @@ void launch_centre_finish(int nsph, int nslot, const double *part, const double 
 }
 
 // ---------------------------------------------------------------------------
+// Traction far field of the rigid-body densities rho0 = F/(4 pi R^2) +
+// 3/(8 pi R^4) T x (R n) (BIE/SphereSTKMobMat.cpp:174-192) of distant spheres,
+// in closed form at the sphere centres.  Outside its sphere, rho0 of sphere S
+// generates the flow of a Stokeslet F, its degenerate quadrupole (R^2/6) lap G F
+// and a rotlet T (all with the 1/(8 pi) normalisation of the single layer);
+// K'[rho0](x) is that flow's traction on a surface of normal n at x.  With
+// r = x - c:
+//   Stokeslet:   -(3/4pi) r (r.F)(r.n)/r^5
+//   quadrupole:  (R^2/12pi) [-3 (F (r.n) + r (F.n) + n (r.F))/r^5 + 15 (r.F)(r.n) r/r^7]
+//   rotlet:      -(3/8pi) [(T x r)(r.n) + r ((T x r).n)]/r^5
+// The order-p grid sum of the traction kernel equals this to ~(R/d)^(2p+2).
+// Targets are grid points (one per lane); intervals run over sphere indices.
+__global__ __launch_bounds__(64) void k_far_rigid_f64(const double *__restrict__ coords,
+                                                      const double *__restrict__ norms,
+                                                      const double *__restrict__ pos, const double *__restrict__ force,
+                                                      const double *__restrict__ rad, const int *__restrict__ gorder,
+                                                      const FarGroup *__restrict__ groups,
+                                                      const FarInterval *__restrict__ iv, double *__restrict__ out,
+                                                      double coef, int accumulate, int64_t slot_stride) {
+    const FarGroup G = groups[gorder[blockIdx.x]];
+    const int lane = threadIdx.x;
+    const int64_t tid = G.tstart + (lane < G.tcount ? lane : 0);
+    const double x = coords[3 * tid], y = coords[3 * tid + 1], z = coords[3 * tid + 2];
+    const double nx = norms[3 * tid], ny = norms[3 * tid + 1], nz = norms[3 * tid + 2];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    constexpr double cS = -3.0 / (4.0 * kPi), cD = 1.0 / (12.0 * kPi), cR = -3.0 / (8.0 * kPi);
+    for (int64_t k = G.iv_begin; k < G.iv_end; k++) {
+        const bool incl = ((iv[k].mask[0] >> lane) & 1ull) != 0;
+        for (int64_t s = iv[k].begin; s < iv[k].end; s++) {
+            const double rx = x - pos[3 * s], ry = y - pos[3 * s + 1], rz = z - pos[3 * s + 2];
+            const double Fx = force[6 * s], Fy = force[6 * s + 1], Fz = force[6 * s + 2];
+            const double Tx = force[6 * s + 3], Ty = force[6 * s + 4], Tz = force[6 * s + 5];
+            const double R2 = rad[s] * rad[s];
+            double ri = rsqrt_f64(fmad(rx, rx, fmad(ry, ry, rz * rz)));
+            ri = incl ? ri : 0.0;
+            const double ri2 = ri * ri, ri5 = ri2 * ri2 * ri;
+            const double rn = fmad(rx, nx, fmad(ry, ny, rz * nz));
+            const double rF = fmad(rx, Fx, fmad(ry, Fy, rz * Fz));
+            const double Fn = fmad(Fx, nx, fmad(Fy, ny, Fz * nz));
+            const double qx = Ty * rz - Tz * ry, qy = Tz * rx - Tx * rz, qz = Tx * ry - Ty * rx; // T x r
+            const double qn = fmad(qx, nx, fmad(qy, ny, qz * nz));
+            // coefficients of r, F, n and T x r
+            const double a_r = (cS * rF * rn + cD * R2 * 15.0 * rF * rn * ri2 - cD * R2 * 3.0 * Fn + cR * qn) * ri5;
+            const double a_F = -cD * R2 * 3.0 * rn * ri5;
+            const double a_n = -cD * R2 * 3.0 * rF * ri5;
+            const double a_q = cR * rn * ri5;
+            ax += a_r * rx + a_F * Fx + a_n * nx + a_q * qx;
+            ay += a_r * ry + a_F * Fy + a_n * ny + a_q * qy;
+            az += a_r * rz + a_F * Fz + a_n * nz + a_q * qz;
+        }
+    }
+    if (lane >= G.tcount) return;
+    if (!accumulate) out += G.slot * slot_stride;
+    double *o = out + 3 * (G.tstart + lane);
+    if (accumulate) {
+        o[0] += coef * ax;
+        o[1] += coef * ay;
+        o[2] += coef * az;
+    } else {
+        o[0] = coef * ax;
+        o[1] = coef * ay;
+        o[2] = coef * az;
+    }
+}
+
+void launch_far_rigid(const double *coords, const double *norms, const double *pos, const double *force,
+                      const double *rad, int64_t ntrg, int ngroups, const int *go, const FarGroup *gr,
+                      const FarInterval *iv, double *out, double coef, int acc, hipStream_t s) {
+    if (ngroups > 0)
+        k_far_rigid_f64<<<dim3(ngroups), dim3(64), 0, s>>>(coords, norms, pos, force, rad, go, gr, iv, out, coef, acc,
+                                                          3 * ntrg);
+}
+
+// ---------------------------------------------------------------------------
 // mobility helpers (SphereSTKMobMat::solveMob, BIE/SphereSTKMobMat.cpp:174-192, :261-285)
 
 // rho0 = F/(4 pi R^2) + 3/(8 pi R^4) T x (R n) on every point of sphere S

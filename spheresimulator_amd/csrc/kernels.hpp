@@ -129,6 +129,11 @@ void launch_axpby(int64_t n, double a, const double *x, double b, double *y, hip
 void launch_fill(int64_t n, double v, double *y, hipStream_t s);
 
 // sphere-centre far field (Faxen) of packed FP64 sources, 9 sums per centre and slot
+// out (+)= coef K'[rho0] at grid points from distant spheres' rigid-body densities, in closed form (Stokeslet,
+// degenerate quadrupole and rotlet at the centres); force holds (F, T) per sphere
+void launch_far_rigid(const double *coords, const double *norms, const double *pos, const double *force,
+                      const double *rad, int64_t ntrg, int ngroups, const int *go, const FarGroup *gr,
+                      const FarInterval *iv, double *out, double coef, int acc, hipStream_t s);
 void launch_far_centre(const double *src, const double *centres, int ngroups, const int *go, const FarGroup *gr,
                        const FarInterval *iv, double *part, int64_t ntrg, hipStream_t s);
 void launch_centre_finish(int nsph, int nslot, const double *part, const double *rad, const double *vel_ns,
