@@ -1,0 +1,65 @@
+"""XYZ files and the Euler step around the solve (CPU)."""
+import os
+
+import numpy as np
+
+import spheresimulator_amd as S
+from conftest import GOLDEN, read_xyz
+
+
+def test_read_xyz_reference_file():
+    """The reference's own input file (posInitial.txt: 100 spheres, R = 1)."""
+    gid, rad, pos = S.system.read_xyz(os.path.join(GOLDEN, "posInitial.txt"))
+    assert len(gid) == 100 and np.array_equal(gid, np.arange(100))
+    assert np.all(rad == 1.0)
+    assert np.array_equal(pos[0], [4.5674967764425, 10.024889032967, 9.9563757054976])
+    p_ref, r_ref = read_xyz(os.path.join(GOLDEN, "posInitial.txt"))
+    assert np.array_equal(pos, p_ref) and np.array_equal(rad, r_ref)
+
+
+def test_write_read_round_trip(tmp_path):
+    rng = np.random.default_rng(1)
+    gid = np.arange(7) * 3 + 1
+    rad = rng.uniform(0.5, 1.5, 7)
+    pos = rng.uniform(-50, 50, (7, 3))
+    f = tmp_path / "s.xyz"
+    S.system.write_xyz(f, gid, rad, pos, time=0.02)
+    lines = open(f).read().splitlines()
+    assert lines[0] == "7" and lines[1] == "SphereNumber: 7, time: 0.020000"
+    assert lines[2].split("\t")[0] == "S"
+    g2, r2, p2 = S.system.read_xyz(f)
+    assert np.array_equal(g2, gid)
+    # 14 significant digits, as the reference writes them
+    assert np.abs(r2 - rad).max() <= 1e-13 * np.abs(rad).max()
+    assert np.abs(p2 - pos).max() <= 1e-13 * np.abs(pos).max()
+
+
+def _rotate(q, v):
+    w, u = q[0], q[1:]
+    t = 2 * np.cross(u, v)
+    return v + w * t + np.cross(u, t)
+
+
+def test_euler_step_rotates_by_omega_dt():
+    rng = np.random.default_rng(2)
+    n = 5
+    pos = rng.standard_normal((n, 3))
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    vel = rng.standard_normal((n, 6))
+    vel[0, 3:] = 0.0  # no rotation
+    dt = 0.37
+    p1, q1 = S.system.step_euler(pos, q, vel, dt)
+    assert np.allclose(p1, pos + dt * vel[:, :3], rtol=0, atol=1e-15)
+    assert np.array_equal(q1[0], q[0] / np.linalg.norm(q[0]))
+    v = rng.standard_normal(3)
+    for i in range(n):
+        om = vel[i, 3:]
+        w = np.linalg.norm(om)
+        b = _rotate(q[i], v)
+        if w > 0:  # Rodrigues: turn by |om| dt about om
+            k = om / w
+            th = w * dt
+            b = b * np.cos(th) + np.cross(k, b) * np.sin(th) + k * (k @ b) * (1 - np.cos(th))
+        assert np.allclose(_rotate(q1[i], v), b, rtol=0, atol=1e-13)
+        assert abs(np.linalg.norm(q1[i]) - 1) < 1e-15
