@@ -503,17 +503,18 @@ Runs runs_of(const std::vector<int> &lst, int s0, int s1, bool include) {
     return r;
 }
 
-// Runs a parallel loop over [0, n) on the host's cores (at most 16 threads).
-static void parallel_for(int n, const std::function<void(int, int)> &body) {
+// Runs a parallel loop over [0, n) on the host's cores (at most 16 threads),
+// handing out `chunk` indices at a time (0: automatic); serial below `serial_below`.
+static void parallel_for(int n, const std::function<void(int, int)> &body, int chunk = 0, int serial_below = 256) {
     const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    if (n < 256 || nt == 1) {
+    if (n < serial_below || nt == 1) {
         body(0, n);
         return;
     }
+    if (chunk <= 0) chunk = std::max(16, n / (8 * nt));
     std::vector<std::thread> th;
     std::atomic<int> next{0};
-    const int chunk = std::max(16, n / (8 * nt));
-    for (int k = 0; k < nt; k++)
+    for (int k = 0; k < std::min(nt, (n + chunk - 1) / chunk); k++)
         th.emplace_back([&] {
             for (;;) {
                 const int a = next.fetch_add(chunk);
@@ -567,35 +568,56 @@ void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sp
     L.tpl = tpl;
     L.split = split;
     L.ntrg = ntrg;
-    std::vector<FarGroup> groups;
-    std::vector<FarInterval> ivs, giv;
-    std::vector<int64_t> work;
-    std::vector<double> cen;
-    double pairs = 0.0;
-    Mask full;
-    full.fill(~0ull);
-    std::vector<std::pair<int, Mask>> ev;
-    // groups inside one sphere all get that sphere's runs with full masks, so
-    // consecutive groups of the same sphere share one copy of the intervals
-    int last_T = -1;
-    int64_t last_work = 0;
-    std::vector<std::array<int64_t, 3>> last_slots; // iv_begin, iv_end, quota per slot
+    // the groups (first target, count, suspension), then their interval lists
+    // built in parallel over chunks of consecutive groups and concatenated
+    struct GSpec {
+        int64_t t0;
+        int cnt, b;
+    };
+    std::vector<GSpec> gs;
     for (int b = 0; b < nsusp; b++) {
         const int64_t p0 = toff[susp_sph_off[b]], p1 = toff[susp_sph_off[b + 1]];
         for (int64_t t0 = p0; t0 < p1;) {
             const int64_t tend = aligned ? toff[tsph[t0] + 1] : p1;
             const int cnt = (int)std::min<int64_t>(G, tend - t0);
+            gs.push_back(GSpec{t0, cnt, b});
+            t0 += cnt;
+        }
+    }
+    struct Part {
+        std::vector<FarGroup> groups;
+        std::vector<FarInterval> ivs;
+        std::vector<int64_t> work;
+        std::vector<double> cen;
+        double pairs = 0.0;
+    };
+    const int nparts = (int)std::min<size_t>(64, std::max<size_t>(1, gs.size() / 512));
+    std::vector<Part> parts(nparts);
+    Mask full;
+    full.fill(~0ull);
+    auto build_part = [&](int pi) {
+        Part &P = parts[pi];
+        const size_t g0 = gs.size() * pi / nparts, g1 = gs.size() * (pi + 1) / nparts;
+        std::vector<FarInterval> giv;
+        std::vector<std::pair<int, Mask>> ev;
+        // groups inside one sphere all get that sphere's runs with full masks, so
+        // consecutive groups of the same sphere share one copy of the intervals
+        int last_T = -1;
+        int64_t last_work = 0;
+        std::vector<std::array<int64_t, 3>> last_slots; // iv_begin, iv_end, quota per slot
+        for (size_t gi = g0; gi < g1; gi++) {
+            const int64_t t0 = gs[gi].t0;
+            const int cnt = gs[gi].cnt, b = gs[gi].b;
             if (tsph[t0] == tsph[t0 + cnt - 1] && tsph[t0] == last_T) {
                 for (int sl = 0; sl < split; sl++) {
-                    groups.push_back(FarGroup{t0, last_slots[sl][0], last_slots[sl][1], cnt, b, sl, 0});
-                    work.push_back(last_slots[sl][2]);
+                    P.groups.push_back(FarGroup{t0, last_slots[sl][0], last_slots[sl][1], cnt, b, sl, 0});
+                    P.work.push_back(last_slots[sl][2]);
                     if (aligned) {
-                        for (int c = 0; c < 3; c++) cen.push_back(sph_pos[3 * last_T + c]);
-                        cen.push_back(0.0);
+                        for (int c = 0; c < 3; c++) P.cen.push_back(sph_pos[3 * last_T + c]);
+                        P.cen.push_back(0.0);
                     }
                 }
-                pairs += double(last_work) * cnt;
-                t0 += cnt;
+                P.pairs += double(last_work) * cnt;
                 continue;
             }
             Mask act{};
@@ -617,7 +639,7 @@ void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sp
                     std::memcpy(iv.mask, m.data(), sizeof(uint64_t) * kMaxTPL);
                     giv.push_back(iv);
                 }
-                pairs += double(ee - bb) * lanes;
+                P.pairs += double(ee - bb) * lanes;
                 gwork += ee - bb;
             };
             const int Ta = tsph[t0], Tb = tsph[t0 + cnt - 1];
@@ -665,14 +687,14 @@ void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sp
             int64_t iv_pos = giv.empty() ? 0 : giv[0].begin;
             for (int sl = 0; sl < split; sl++) {
                 const int64_t quota = gwork * (sl + 1) / split - gwork * sl / split;
-                FarGroup g{t0, (int64_t)ivs.size(), 0, cnt, b, sl, 0};
+                FarGroup g{t0, (int64_t)P.ivs.size(), 0, cnt, b, sl, 0};
                 int64_t left = quota;
                 while (left > 0 && iv_i < giv.size()) {
                     const int64_t take = std::min(giv[iv_i].end - iv_pos, left);
                     FarInterval piece = giv[iv_i];
                     piece.begin = iv_pos;
                     piece.end = iv_pos + take;
-                    ivs.push_back(piece);
+                    P.ivs.push_back(piece);
                     left -= take;
                     iv_pos += take;
                     if (iv_pos == giv[iv_i].end) {
@@ -680,17 +702,50 @@ void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sp
                         if (iv_i < giv.size()) iv_pos = giv[iv_i].begin;
                     }
                 }
-                g.iv_end = (int64_t)ivs.size();
-                groups.push_back(g);
-                work.push_back(quota);
+                g.iv_end = (int64_t)P.ivs.size();
+                P.groups.push_back(g);
+                P.work.push_back(quota);
                 last_slots[sl] = {g.iv_begin, g.iv_end, quota};
                 if (aligned) {
-                    const int T = tsph[t0];
-                    for (int c = 0; c < 3; c++) cen.push_back(sph_pos[3 * T + c]);
-                    cen.push_back(0.0);
+                    for (int c = 0; c < 3; c++) P.cen.push_back(sph_pos[3 * Ta + c]);
+                    P.cen.push_back(0.0);
                 }
             }
-            t0 += cnt;
+        }
+    };
+    parallel_for(
+        nparts,
+        [&](int a, int e) {
+            for (int pi = a; pi < e; pi++) build_part(pi);
+        },
+        1, 2);
+    std::vector<FarGroup> groups;
+    std::vector<FarInterval> ivs;
+    std::vector<int64_t> work;
+    std::vector<double> cen;
+    double pairs = 0.0;
+    {
+        size_t ng = 0, ni = 0;
+        for (const Part &P : parts) {
+            ng += P.groups.size();
+            ni += P.ivs.size();
+        }
+        groups.reserve(ng);
+        ivs.reserve(ni);
+        work.reserve(ng);
+        for (Part &P : parts) {
+            const int64_t base = (int64_t)ivs.size();
+            for (FarGroup g : P.groups) {
+                g.iv_begin += base;
+                g.iv_end += base;
+                groups.push_back(g);
+            }
+            ivs.insert(ivs.end(), P.ivs.begin(), P.ivs.end());
+            work.insert(work.end(), P.work.begin(), P.work.end());
+            cen.insert(cen.end(), P.cen.begin(), P.cen.end());
+            pairs += P.pairs;
+            Part().groups.swap(P.groups);
+            std::vector<FarInterval>().swap(P.ivs);
         }
     }
     for (const FarGroup &g : groups) {
