@@ -283,7 +283,7 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     if (const char *v = std::getenv("SPHSTK_FAR_SPLIT")) split = std::atoi(v);
     // FP32 inner operator: orders from level_order(); FP64 outer applies: fine
     // below 8 (R_s + R_t), then orders 9, 7, 6 from 8, 16, 32 (R_s + R_t).
-    ls32_.alo[0] = 4.0, ls32_.alo[1] = 8.0, ls32_.alo[2] = 14.0, ls32_.alo[3] = 28.0;
+    ls32_.alo[0] = 4.0, ls32_.alo[1] = 8.0, ls32_.alo[2] = 14.0, ls32_.alo[3] = 22.0;
     ls32_.order[0] = ls32_.order[1] = ls32_.order[2] = ls32_.order[3] = -1;
     ls32_.fp64 = false;
     ls64_.alo[0] = 8.0, ls64_.alo[1] = 16.0, ls64_.alo[2] = 32.0, ls64_.alo[3] = 64.0;
