@@ -1688,7 +1688,13 @@ void Mobility::solve(const double *force, double *vel, std::vector<SegInfo> &inf
             std::vector<SegInfo> inner(B);
             launch_fill(n, 0.0, d_.p, st);
             std::vector<int> ia = a2;
-            const double itol = std::max(prm_.inner_tol, prm_.tol);
+            // inexact inner solve: a refinement step only has to take the
+            // residual below the target, so its tolerance is the larger of
+            // inner_tol and a tenth of tol / (current relative residual)
+            double worst = 0.0;
+            for (int s = 0; s < B; s++)
+                if (a2[s]) worst = std::max(worst, info[s].relres);
+            const double itol = std::max({prm_.inner_tol, prm_.tol, worst > 0 ? 0.1 * prm_.tol / worst : 0.0});
             // the inner solve overwrites r_, so its right-hand side is a copy of the outer residual
             gmres_inner_rhs_.alloc(n);
             launch_axpby(n, 1.0, r_.p, 0.0, gmres_inner_rhs_.p, st);

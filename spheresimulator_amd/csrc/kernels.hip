@@ -1,16 +1,20 @@
 // HIP kernels of the many-sphere Stokes boundary-integral operator (gfx950).
 //
-// Far field   k_far_*        direct Stokeslet / stresslet pair sums over the
-//                            far pairs of every target (replaces the STKFMM
+// Far field   k_far_f64/_lvl direct Stokeslet / stresslet pair sums over the
+//             k_far_f32      far pairs of every target (replaces the STKFMM
 //                            evaluation, BIE/SphereSTKSHOperator.cpp:287-303,
-//                            :365-446).  One 64-lane wave per group of 64*TPL
-//                            targets; sources are wave-uniform and reach the
-//                            VALU through scalar loads (no LDS staging).
-// SH analysis k_analysis     grid values -> vector-SH coefficients per sphere
-//                            (Shexp::calcSpectralCoeff, Sphere/Shexp.cpp:408-447).
+//                            :365-446), over the fine grid or a level's grids.
+//                            One 64-lane wave per group of 64*TPL targets;
+//                            sources are wave-uniform and reach the VALU
+//                            through scalar loads (no LDS staging).
+// Centre sums k_far_centre   Faxen sphere-centre sums of the final velocities;
+//             k_far_rigid    closed-form traction of distant rigid-body densities.
+// SH matrices k_tiled_apply  grid values -> vector-SH coefficients per sphere
+//                            (Shexp::calcSpectralCoeff, Sphere/Shexp.cpp:408-447)
+//                            and resampling between grids, 16 spheres per tile.
 // Expansion   k_expand       coefficients -> exterior solid-harmonic expansion
 //                            records (the flow of SCTL StokesEvalSL/KL/KSelf).
-// Near/self   k_nearself     self (SphereSTKSHOperator.cpp:305-351, :375-456)
+// Near/self   k_sheval       self (SphereSTKSHOperator.cpp:305-351, :375-456)
 //                            and near-pair (:575-695) SH evaluations.
 // LOP         k_lop          rigid-body projector (:487-521).
 // BLAS-1      k_seg_*        segmented (one segment per suspension) GMRES
