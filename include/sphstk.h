@@ -136,6 +136,12 @@ int sphstk_operator_apply_p2p_device(sphstk_operator *op, const double *din, dou
 /* applyLOP: out += cLOP L[in]  (cpp:487-521; hpp:208-210). */
 int sphstk_operator_apply_lop(sphstk_operator *op, const double *in, double *out, double cLOP);
 
+/* Values of the spherical-harmonic expansion of the grid field `in` (ndof) at
+ * the north and south poles of every sphere's SH frame, lab frame: poles[6 S +
+ * 0..2] north, [6 S + 3..5] south (Shexp::calcPoleValue, Shexp.cpp:382-405,
+ * which feeds the grid-with-poles output of Shexp::writeVTU, :96-204). */
+int sphstk_operator_pole_values(sphstk_operator *op, const double *in, double *poles);
+
 /* projectNullSpace: in-place projection onto vector SH of degree <= p
  * (cpp:239-260; hpp:200-202). */
 int sphstk_operator_project_nullspace(sphstk_operator *op, double *inout);

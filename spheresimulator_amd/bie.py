@@ -176,6 +176,13 @@ class SphereSTKSHOperator:
     def reset_stats(self):
         check(load().sphstk_operator_reset_stats(self._h))
 
+    def pole_values(self, x):
+        """SH expansion of the grid field x at each sphere's two poles (lab frame), shape (N, 2, 3)."""
+        x = _f64(x, self.n_dof, "x")
+        out = np.zeros(6 * self.n_spheres)
+        check(load().sphstk_operator_pole_values(self._h, _d(x), _d(out)))
+        return out.reshape(-1, 2, 3)
+
     def far_shape(self):
         """Far-field work-list shapes in use: FP64 targets per lane and source slices, FP32 targets per lane."""
         t, s, i = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

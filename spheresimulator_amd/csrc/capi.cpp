@@ -236,6 +236,19 @@ int sphstk_operator_apply_lop(sphstk_operator *op, const double *in, double *out
     });
 }
 
+int sphstk_operator_pole_values(sphstk_operator *op, const double *in, double *poles) {
+    return guard([&] {
+        SPHSTK_REQUIRE(op && op->op, "invalid operator handle");
+        SPHSTK_REQUIRE(in && poles, "in and poles must not be NULL");
+        Operator &A = *op->op;
+        const int64_t n = A.ndof();
+        DevBuf<double> d;
+        d.alloc(std::max<int64_t>(n, 1));
+        SPHSTK_HIP(hipMemcpyAsync(d.p, in, sizeof(double) * n, hipMemcpyHostToDevice, A.stream()));
+        A.pole_values(d.p, poles);
+    });
+}
+
 int sphstk_operator_project_nullspace(sphstk_operator *op, double *inout) {
     return guard([&] {
         SPHSTK_REQUIRE(op && op->op, "invalid operator handle");

@@ -73,6 +73,10 @@ struct ShTables {
 
 const ShTables &sh_tables(int p);
 
+// The synthesis rows at the body-frame poles +z and -z: 6 rows (north x, y, z,
+// south x, y, z) of 6Q coefficients each, row-major.
+std::vector<double> pole_rows(int p);
+
 // Per-order resampling matrices between the order-p grid and a coarser grid of
 // order pc <= p, both acting on body-frame values (3 per point), column-major:
 //   down (3 nc x 3 n): coarse values of the degree-pc truncation of the projection

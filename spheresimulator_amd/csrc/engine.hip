@@ -1329,6 +1329,36 @@ void Operator::apply(const double *X, double *Y, double alpha, double beta, int 
 }
 
 // projectNullSpace (cpp:239-260)
+// Values of the SH expansion of `in` (device DOF vector) at every sphere's two
+// poles, lab frame: out[6 S + (0..2 north, 3..5 south)] (Shexp::calcPoleValue,
+// Shexp.cpp:382-405; used by the grid-with-poles output).  The coefficients
+// come from the device analysis; the 6 synthesis rows per order are evaluated
+// on the host.
+void Operator::pole_values(const double *in, double *out) {
+    if (nsph_ == 0) return;
+    launch_analysis(geom_, in, coef_.p, nullptr, stream_, max_order_);
+    std::vector<double> coef(std::max<int64_t>(coef_off_[nsph_], 1));
+    SPHSTK_HIP(hipMemcpyAsync(coef.data(), coef_.p, sizeof(double) * coef_off_[nsph_], hipMemcpyDeviceToHost,
+                              stream_));
+    SPHSTK_HIP(hipStreamSynchronize(stream_));
+    std::vector<std::vector<double>> rows(kMaxOrder + 1);
+    for (int S = 0; S < nsph_; S++) {
+        const int p = order_[S], ncoef = 6 * tri_size(p);
+        if (rows[p].empty()) rows[p] = pole_rows(p);
+        const double *c = coef.data() + coef_off_[S];
+        for (int pole = 0; pole < 2; pole++) {
+            double body[3];
+            for (int k = 0; k < 3; k++) {
+                const double *r = rows[p].data() + size_t(3 * pole + k) * ncoef;
+                double s = 0.0;
+                for (int j = 0; j < ncoef; j++) s += r[j] * c[j];
+                body[k] = s;
+            }
+            qrot_host(&quat_[4 * S], body, out + 6 * S + 3 * pole);
+        }
+    }
+}
+
 void Operator::project(double *inout) {
     if (nsph_ == 0) return;
     launch_analysis(geom_, inout, coef_.p, nullptr, stream_, max_order_);

@@ -157,6 +157,8 @@ class Operator {
     // from a HIP graph captured on the first call for this `active` array
     void apply_inner_into(const double *in, double *out, const int *active);
     void project(double *inout);
+    // SH expansion of in (device) at the two poles of every sphere, lab frame, 6 per sphere (host out)
+    void pole_values(const double *in, double *out);
     // rigid-body velocities (6 per sphere) of the single layer of rho, distant pairs by Faxen's laws
     void rigid_velocity(const double *rho, double inv_mu, double *vel);
     // builds the level sets and final-velocity lists of the mixed-precision solve now

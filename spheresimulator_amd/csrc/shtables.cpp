@@ -77,6 +77,54 @@ static void solid_harmonics(int P, const double *x, std::vector<cplx> &D) {
     }
 }
 
+// Rows of the synthesis operator at one unit vector e (body frame): the value
+// component c of the expansion at e is sum_j row[c * 6Q + j] coef[j].
+static void synthesis_rows_at(int p, const double *e, std::vector<cplx> &D, double *row) {
+    const int Q = tri_size(p), ncoef = 6 * Q;
+    std::fill(row, row + 3 * ncoef, 0.0);
+    solid_harmonics(p + 1, e, D);
+    for (int n = 0; n <= p; n++)
+        for (int m = 0; m <= n; m++) {
+            const int k = tri(n, m);
+            cplx g[3];
+            g[2] = D[tri(n + 1, m)];
+            if (m > 0) {
+                g[0] = 0.5 * (D[tri(n + 1, m + 1)] - D[tri(n + 1, m - 1)]);
+                g[1] = cplx(0, -0.5) * (D[tri(n + 1, m + 1)] + D[tri(n + 1, m - 1)]);
+            } else {
+                g[0] = D[tri(n + 1, 1)].real();
+                g[1] = D[tri(n + 1, 1)].imag();
+            }
+            const cplx y = D[k];
+            cplx gs[3], xt[3];
+            for (int c = 0; c < 3; c++) gs[c] = g[c] + (n + 1.0) * y * e[c];
+            xt[0] = e[1] * g[2] - e[2] * g[1];
+            xt[1] = e[2] * g[0] - e[0] * g[2];
+            xt[2] = e[0] * g[1] - e[1] * g[0];
+            const double eps = m ? 2.0 : 1.0;
+            for (int c = 0; c < 3; c++) {
+                double *rc = row + size_t(c) * ncoef;
+                const cplx br = y * e[c], bs = gs[c], bt = xt[c];
+                rc[2 * k] = eps * br.real();
+                rc[2 * k + 1] = -eps * br.imag();
+                rc[2 * Q + 2 * k] = eps * bs.real();
+                rc[2 * Q + 2 * k + 1] = -eps * bs.imag();
+                rc[4 * Q + 2 * k] = eps * bt.real();
+                rc[4 * Q + 2 * k + 1] = -eps * bt.imag();
+            }
+        }
+}
+
+std::vector<double> pole_rows(int p) {
+    const int ncoef = 6 * tri_size(p);
+    std::vector<double> rows(size_t(6) * ncoef);
+    std::vector<cplx> D;
+    const double north[3] = {0.0, 0.0, 1.0}, south[3] = {0.0, 0.0, -1.0};
+    synthesis_rows_at(p, north, D, rows.data());
+    synthesis_rows_at(p, south, D, rows.data() + size_t(3) * ncoef);
+    return rows;
+}
+
 static std::unique_ptr<ShTables> build_tables(int p) {
     auto t = std::make_unique<ShTables>();
     t->p = p;

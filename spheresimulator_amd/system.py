@@ -70,3 +70,93 @@ def step_euler(pos, quat, vel, dt):
     q = _qmul(dq, quat)
     q /= np.linalg.norm(q, axis=1)[:, None]
     return pos + vel[:, :3] * dt, q
+
+
+def grid_with_poles_cells(p):
+    """Cells of one sphere's grid with poles (Shexp::getGridWithPoleCellConnect,
+    Sphere/Shexp.cpp:273-380): point 0 is the north pole, then the (p+1) rings
+    of 2p+2 grid points from north to south, then the south pole; triangles
+    (VTK type 5) fan around the poles, quads (type 9) join neighbouring rings.
+    Returns (connectivity, offsets (end of each cell), types)."""
+    nphi = 2 * p + 2
+    south = 2 * p * p + 4 * p + 3
+    conn, off, typ = [], [], []
+
+    def cell(nodes, t):
+        conn.extend(nodes)
+        off.append(len(conn))
+        typ.append(t)
+
+    index = 0
+    for _ in range(2 * p + 1):
+        index += 1
+        cell([0, index, index + 1], 5)
+    index += 1
+    cell([0, nphi, 1], 5)
+    for _ in range(1, p + 1):
+        for _ in range(2 * p + 1):
+            index += 1
+            cell([index, index + 1, index + 1 - nphi, index - nphi], 9)
+        index += 1
+        cell([index, index - (2 * p + 1), index - (2 * p + 1) - nphi, index - nphi], 9)
+    cell([south, south - 1, south - 1 - (2 * p + 1)], 5)
+    for _ in range(2 * p + 1):
+        index -= 1
+        cell([south, index + 1, index], 5)
+    return np.array(conn, dtype=np.int32), np.array(off, dtype=np.int32), np.array(typ, dtype=np.uint8)
+
+
+def _qrot(q, v):
+    w, u = q[0], np.asarray(q[1:])
+    t = 2.0 * np.cross(u, v)
+    return v + w * t + np.cross(u, t)
+
+
+def _array(f, name, a, ncomp, vtype):
+    f.write('<DataArray type="%s" Name="%s" NumberOfComponents="%d" format="ascii">\n' % (vtype, name, ncomp))
+    fmt = "%d" if vtype.startswith(("Int", "UInt")) else "%.17g"
+    f.write(" ".join(fmt % x for x in np.asarray(a).ravel()))
+    f.write("\n</DataArray>\n")
+
+
+def write_vtu(path, op, pos, radius, quat, order, density, name="stkmob"):
+    """Grid density of every sphere as a VTK unstructured grid, one Piece per
+    sphere (SphereSystem::writeVTK -> Shexp::writeVTU, Sphere/Shexp.cpp:96-204):
+    the points with poles (north pole, the grid, south pole; Shexp.cpp:206-271),
+    the density (3 components, the grid values as given and the poles from the
+    SH expansion, `op.pole_values`), the quadrature weights (0 at the poles) and
+    the triangle/quad cells of grid_with_poles_cells.  `op` is the
+    SphereSTKSHOperator of these spheres.  ASCII data arrays."""
+    pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+    radius = np.asarray(radius, dtype=np.float64).ravel()
+    quat = np.asarray(quat, dtype=np.float64).reshape(-1, 4)
+    order = np.asarray(order).ravel()
+    density = np.asarray(density, dtype=np.float64).ravel()
+    g = op.grid()
+    poles = op.pole_values(density)
+    coords = g["coords"].reshape(-1, 3)
+    weights = g["weights"]
+    dens = density.reshape(-1, 3)
+    with open(path, "w") as f:
+        f.write('<?xml version="1.0"?>\n<VTKFile type="UnstructuredGrid" version="0.1" byte_order="LittleEndian">\n')
+        f.write("<UnstructuredGrid>\n")
+        for s in range(len(radius)):
+            p = int(order[s])
+            a, n = int(g["index"][s]), int(g["length"][s])
+            ez = _qrot(quat[s], np.array([0.0, 0.0, 1.0]))
+            pts = np.vstack([pos[s] + radius[s] * ez, coords[a:a + n], pos[s] - radius[s] * ez])
+            val = np.vstack([poles[s, 0], dens[a:a + n], poles[s, 1]])
+            w = np.concatenate([[0.0], weights[a:a + n], [0.0]])
+            conn, off, typ = grid_with_poles_cells(p)
+            f.write('<Piece NumberOfPoints="%d" NumberOfCells="%d">\n' % (len(pts), len(typ)))
+            f.write('<PointData Scalars="scalars">\n')
+            _array(f, name, val, 3, "Float64")
+            _array(f, "weights", w, 1, "Float64")
+            f.write("</PointData>\n<Points>\n")
+            _array(f, "points", pts, 3, "Float64")
+            f.write("</Points>\n<Cells>\n")
+            _array(f, "connectivity", conn, 1, "Int32")
+            _array(f, "offsets", off, 1, "Int32")
+            _array(f, "types", typ, 1, "UInt8")
+            f.write("</Cells>\n</Piece>\n")
+        f.write("</UnstructuredGrid>\n</VTKFile>\n")

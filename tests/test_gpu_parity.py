@@ -324,3 +324,46 @@ def test_inner_apply_graph_replays_match_eager():
     v0, i0 = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], **kw).solveMob(c["force"])
     v1, i1 = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], graph=True, **kw).solveMob(c["force"])
     assert i1["converged"] == 1 and i1["iters"] == i0["iters"] and rel(v1, v0) <= 1e-13
+
+
+def test_pole_values_and_vtu_output(tmp_path):
+    """Shexp::calcPoleValue and writeVTU: for a band-limited grid field (a
+    constant, a rigid rotation c x n and a cubic in n) the expansion at the two
+    poles of each rotated SH frame equals the field there; the VTU file holds
+    one piece per sphere with the poles first and last, zero pole weights and
+    the grid weights summing to 4 pi R^2."""
+    import xml.etree.ElementTree as ET
+    say("pole values and VTU output")
+    pos, rad, quat, order = small_geometry(seed=51, n=4, orders=(8, 6, 10, 7))
+    op = S.SphereSTKSHOperator(pos, rad, quat, order)
+    g = op.grid()
+    nv = g["norms"].reshape(-1, 3)
+    a = np.array([0.3, -0.7, 1.1])
+    c = np.array([0.4, 0.2, -0.5])
+
+    def field(n):
+        return a + np.cross(c, n) + np.stack([n[:, 0] * n[:, 1] * n[:, 2], n[:, 0] ** 2, n[:, 2] ** 3 - n[:, 1]], 1)
+
+    x = field(nv).ravel()
+    poles = op.pole_values(x)
+    for s in range(len(rad)):
+        w, u = quat[s, 0], quat[s, 1:]
+        t = 2 * np.cross(u, [0.0, 0.0, 1.0])
+        ez = np.array([0.0, 0.0, 1.0]) + w * t + np.cross(u, t)
+        want = field(np.stack([ez, -ez]))
+        assert np.abs(poles[s] - want).max() < 1e-12
+    f = tmp_path / "density.vtu"
+    S.system.write_vtu(f, op, pos, rad, quat, order, x)
+    pieces = ET.parse(f).getroot().find("UnstructuredGrid").findall("Piece")
+    assert len(pieces) == len(rad)
+    for s, piece in enumerate(pieces):
+        p = int(order[s])
+        assert int(piece.get("NumberOfPoints")) == 2 * p * p + 4 * p + 4
+        arr = {d.get("Name"): np.array(d.text.split(), float) for d in piece.iter("DataArray")}
+        vals = arr["stkmob"].reshape(-1, 3)
+        assert np.allclose(vals[0], poles[s, 0], rtol=0, atol=1e-14)
+        assert np.allclose(vals[-1], poles[s, 1], rtol=0, atol=1e-14)
+        w = arr["weights"]
+        assert w[0] == 0.0 and w[-1] == 0.0 and abs(w.sum() - 4 * np.pi * rad[s] ** 2) < 1e-12
+        pts = arr["points"].reshape(-1, 3)
+        assert np.allclose(np.linalg.norm(pts - pos[s], axis=1), rad[s], rtol=0, atol=1e-13)

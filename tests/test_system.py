@@ -63,3 +63,27 @@ def test_euler_step_rotates_by_omega_dt():
             b = b * np.cos(th) + np.cross(k, b) * np.sin(th) + k * (k @ b) * (1 - np.cos(th))
         assert np.allclose(_rotate(q1[i], v), b, rtol=0, atol=1e-13)
         assert abs(np.linalg.norm(q1[i]) - 1) < 1e-15
+
+
+def test_grid_with_poles_cells_cover_the_sphere():
+    """The cells of Shexp::getGridWithPoleCellConnect: (2p+2)(p+2) cells over
+    2p^2+4p+4 points, every point used, every edge shared by exactly two cells
+    (a closed surface), triangles only at the poles."""
+    for p in (1, 4, 8, 12):
+        conn, off, typ = S.system.grid_with_poles_cells(p)
+        npts = 2 * p * p + 4 * p + 4
+        assert len(typ) == (2 * p + 2) * (p + 2)
+        assert conn.min() == 0 and conn.max() == npts - 1 and len(np.unique(conn)) == npts
+        assert off[-1] == len(conn) and np.all(np.diff(np.concatenate([[0], off])) == np.where(typ == 5, 3, 4))
+        assert np.sum(typ == 5) == 2 * (2 * p + 2) and np.sum(typ == 9) == p * (2 * p + 2)
+        edges = {}
+        start = 0
+        for e in off:
+            cell = list(conn[start:e])
+            for a, b in zip(cell, cell[1:] + cell[:1]):
+                key = (min(a, b), max(a, b))
+                edges[key] = edges.get(key, 0) + 1
+            start = e
+        assert set(edges.values()) == {2}
+        # Euler characteristic of a sphere: V - E + F = 2
+        assert npts - len(edges) + len(typ) == 2
