@@ -1,14 +1,16 @@
-// Host engine: operator set-up (grid, near list, far-field work lists), the
-// operator apply pipeline, batched restarted GMRES and the mobility solve in
-// FP64 and in mixed precision (FP32 far field inside FP64 iterative refinement).
+// Host engine: operator set-up (grid, near list, far-field work lists and the
+// level sets of the multi-level far fields, the tile-shape autotuner), the
+// operator apply pipeline (and its captured HIP graph), batched restarted GMRES
+// and the mobility solve in FP64 and in mixed precision (FP32 multi-level inner
+// operator inside FP64 iterative refinement with multi-level FP64 residuals).
 #include "engine.hpp"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
+#include <chrono>
 #include <functional>
 #include <thread>
-#include <chrono>
-#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
