@@ -24,7 +24,7 @@ EXPORTED = [
     "sphstk_operator_project_nullspace", "sphstk_operator_pole_values", "sphstk_operator_get_grid", "sphstk_operator_get_near_pairs",
     "sphstk_operator_far_shape", "sphstk_operator_kernel_stats", "sphstk_operator_reset_stats",
     "sphstk_mobility_create", "sphstk_mobility_create_batched", "sphstk_mobility_destroy",
-    "sphstk_mobility_operator", "sphstk_mobility_set_params", "sphstk_mobility_solve", "sphstk_mobility_apply",
+    "sphstk_mobility_operator", "sphstk_mobility_set_params", "sphstk_mobility_solve", "sphstk_mobility_solve_multi", "sphstk_mobility_apply",
     "sphstk_mobility_get_density", "sphstk_mobility_get_unknown", "sphstk_mobility_set_unknown",
 ]
 
@@ -94,6 +94,7 @@ def load():
     L.sphstk_mobility_operator.restype = _vp
     L.sphstk_mobility_set_params.argtypes = [_vp, ctypes.POINTER(SolverParams), ctypes.c_int]
     L.sphstk_mobility_solve.argtypes = [_vp, _dp, _dp, ctypes.POINTER(SolveInfo)]
+    L.sphstk_mobility_solve_multi.argtypes = [_vp, ctypes.c_int, _dp, _dp, ctypes.POINTER(SolveInfo)]
     L.sphstk_mobility_apply.argtypes = [_vp, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.sphstk_mobility_get_density.argtypes = [_vp, _dp]
     L.sphstk_mobility_get_unknown.argtypes = [_vp, _dp]

@@ -231,6 +231,14 @@ class SphereSTKMobMat:
         check(load().sphstk_mobility_solve(self._h, _d(force), _d(vel), ctypes.byref(info)))
         return vel, info.as_dict()
 
+    def solveMob_multi(self, force):
+        """Several right-hand sides (ncol, 6N) in one FP64 lock-step solve from x0 = 0; returns (ncol, 6N)."""
+        force = np.ascontiguousarray(np.asarray(force, dtype=np.float64).reshape(-1, 6 * self.n_spheres))
+        vel = np.zeros_like(force)
+        info = SolveInfo()
+        check(load().sphstk_mobility_solve_multi(self._h, force.shape[0], _d(force), _d(vel), ctypes.byref(info)))
+        return vel, info.as_dict()
+
     def apply(self, X, Y=None, alpha=1.0, beta=0.0):
         X = _f64(X)
         n6 = 6 * self.n_spheres

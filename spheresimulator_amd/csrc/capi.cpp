@@ -18,11 +18,35 @@ struct sphstk_mobility {
     std::unique_ptr<Mobility> mob;
     sphstk_operator view;
     std::vector<SegInfo> last;
+    // the creation inputs, kept for the several-right-hand-side solver
+    std::vector<int> counts, order;
+    std::vector<double> pos, radius, quat;
+    double viscosity = 1.0;
+    int flags = 0;
+    std::unique_ptr<Mobility> multi; // copies of the suspensions, one per right-hand side
+    int multi_ncol = 0;
 };
 
 namespace {
 
 thread_local std::string g_err;
+
+// solve info aggregated over the suspensions (iters = max)
+void fill_info(sphstk_solve_info *info, const std::vector<SegInfo> &seg, int outer, int a64, int a32) {
+    if (!info) return;
+    std::memset(info, 0, sizeof(*info));
+    info->converged = 1;
+    info->shortcut = 1;
+    for (const auto &s : seg) {
+        info->iters = std::max(info->iters, s.iters);
+        info->converged = info->converged && s.converged;
+        info->shortcut = info->shortcut && s.shortcut;
+        info->relres = std::max(info->relres, s.relres);
+    }
+    info->outer = outer;
+    info->applies_fp64 = a64;
+    info->applies_fp32 = a32;
+}
 
 int fail(int code, const std::string &msg) {
     g_err = msg;
@@ -321,6 +345,14 @@ int sphstk_mobility_create_batched(int n_susp, const int *sphere_counts, const d
         h->mob = std::make_unique<Mobility>(n_susp, sphere_counts, pos, radius, quat_wxyz, order, viscosity, prm,
                                             precision, flags);
         h->view.op = &h->mob->op();
+        const int n = h->mob->op().nsph();
+        h->counts.assign(sphere_counts, sphere_counts + n_susp);
+        h->order.assign(order, order + n);
+        h->pos.assign(pos, pos + 3 * n);
+        h->radius.assign(radius, radius + n);
+        h->quat.assign(quat_wxyz, quat_wxyz + 4 * n);
+        h->viscosity = viscosity;
+        h->flags = flags;
         *out = h.release();
     });
 }
@@ -345,20 +377,7 @@ int sphstk_mobility_solve(sphstk_mobility *mob, const double *force, double *vel
         SPHSTK_REQUIRE(mob->mob->op().nsph() == 0 || (force && vel), "force and vel must not be NULL");
         int outer = 0, a64 = 0, a32 = 0;
         mob->mob->solve(force, vel, mob->last, outer, a64, a32);
-        if (info) {
-            std::memset(info, 0, sizeof(*info));
-            info->converged = 1;
-            info->shortcut = 1;
-            for (const auto &s : mob->last) {
-                info->iters = std::max(info->iters, s.iters);
-                info->converged = info->converged && s.converged;
-                info->shortcut = info->shortcut && s.shortcut;
-                info->relres = std::max(info->relres, s.relres);
-            }
-            info->outer = outer;
-            info->applies_fp64 = a64;
-            info->applies_fp32 = a32;
-        }
+        fill_info(info, mob->last, outer, a64, a32);
     });
 }
 
@@ -373,6 +392,38 @@ int sphstk_mobility_apply(sphstk_mobility *mob, const double *X, double *Y, int 
             mob->mob->solve(X + c * n6, v.data(), mob->last, outer, a64, a32);
             for (int64_t i = 0; i < n6; i++) Y[c * n6 + i] = (beta == 0.0 ? 0.0 : beta * Y[c * n6 + i]) + alpha * v[i];
         }
+    });
+}
+
+int sphstk_mobility_solve_multi(sphstk_mobility *mob, int ncol, const double *force, double *vel,
+                                sphstk_solve_info *info) {
+    return guard([&] {
+        SPHSTK_REQUIRE(mob && force && vel, "invalid arguments");
+        SPHSTK_REQUIRE(ncol >= 1 && ncol <= 64, "ncol must lie in [1, 64]");
+        SPHSTK_REQUIRE(mob->mob->precision() == 0, "several right-hand sides: FP64 form only");
+        const int B = (int)mob->counts.size();
+        if (!mob->multi || mob->multi_ncol != ncol) {
+            mob->multi.reset();
+            std::vector<int> counts, order;
+            std::vector<double> pos, radius, quat;
+            for (int c = 0; c < ncol; c++) {
+                counts.insert(counts.end(), mob->counts.begin(), mob->counts.end());
+                order.insert(order.end(), mob->order.begin(), mob->order.end());
+                pos.insert(pos.end(), mob->pos.begin(), mob->pos.end());
+                radius.insert(radius.end(), mob->radius.begin(), mob->radius.end());
+                quat.insert(quat.end(), mob->quat.begin(), mob->quat.end());
+            }
+            mob->multi = std::make_unique<Mobility>(ncol * B, counts.data(), pos.data(), radius.data(), quat.data(),
+                                                    order.data(), mob->viscosity, mob->mob->params(), 0,
+                                                    mob->flags & ~4, ncol);
+            mob->multi_ncol = ncol;
+        }
+        mob->multi->set_params(mob->mob->params(), 0);
+        std::vector<SegInfo> seg;
+        int outer = 0, a64 = 0, a32 = 0;
+        mob->multi->set_unknown_zero();
+        mob->multi->solve(force, vel, seg, outer, a64, a32);
+        fill_info(info, seg, outer, a64, a32);
     });
 }
 

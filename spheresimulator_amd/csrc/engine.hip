@@ -66,8 +66,8 @@ static std::vector<Chunk> make_chunks(int nsusp, const std::vector<int64_t> &sus
 // (BIE/SphereSTKSHOperator.cpp:37-144, :539-573).
 
 Operator::Operator(int nsusp, const int *counts, const double *pos, const double *radius, const double *quat,
-                   const int *order, double cId, double cSL, double cTrac, double cLOP, int flags)
-    : nsph_(0), nsusp_(nsusp), npts_(0), flags_(flags) {
+                   const int *order, double cId, double cSL, double cTrac, double cLOP, int flags, int ncopy)
+    : nsph_(0), nsusp_(nsusp), npts_(0), ncopy_(ncopy), flags_(flags) {
     PhaseTimer pt("operator");
     c_[0] = cId;
     c_[1] = cSL;
@@ -75,6 +75,7 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     c_[3] = cLOP;
     SPHSTK_REQUIRE((flags & ~15) == 0, "unknown flags (known: TIMING 1, FP32_DIRECT 2, AUTOTUNE 4, GRAPH 8)");
     SPHSTK_REQUIRE(nsusp >= 1, "number of suspensions must be >= 1");
+    SPHSTK_REQUIRE(ncopy >= 1 && ncopy <= 64 && nsusp % ncopy == 0, "copies must divide the suspensions");
     SPHSTK_REQUIRE(counts != nullptr, "sphere counts must not be NULL");
     susp_sph_off_.assign(nsusp + 1, 0);
     for (int b = 0; b < nsusp; b++) {
@@ -278,6 +279,7 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     const std::vector<Chunk> ch = make_chunks(nsusp_, susp_sph_off_, pt_off_, &sco);
     chunks_.upload(ch);
     seg_chunk_off_.upload(sco);
+    seg_chunk_off_host_ = sco;
     partial_.alloc(std::max<size_t>(ch.size(), 1));
 
     int tpl = 1, split = 0;
@@ -309,6 +311,10 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     multi_level_ = (flags_ & 2) == 0;
     if (const char *v = std::getenv("SPHSTK_MULTI_LEVEL")) multi_level_ = std::atoi(v) != 0;
     if (const char *v = std::getenv("SPHSTK_FAR_MFMA")) far_mfma_ = std::atoi(v) != 0;
+    if (ncopy_ > 1) {
+        far_mfma_ = false;
+        tpl = 1;
+    }
     if (far_mfma_) tpl = 1;
     {
         PhaseTimer pt2("fine far-field lists");
@@ -323,8 +329,27 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
             const int b = sph_susp_[t];
             runs_fine_[t] = runs_of(excl[t], (int)susp_sph_off_[b], (int)susp_sph_off_[b + 1], false);
         }
-        build_far_lists(lists_fine_, nsusp_, susp_sph_off_, pt_off_, pt_off_, runs_fine_, tpl, split, kWantItems,
-                        far_mfma_ ? pos_.data() : nullptr);
+        if (ncopy_ > 1) {
+            // work lists of the first copy only; its geometry serves all copies
+            const int b0 = nsusp_ / ncopy_, s0 = (int)susp_sph_off_[b0];
+            for (int c = 1; c < ncopy_; c++)
+                for (int i = 0; i < s0; i++) {
+                    const int j = c * s0 + i;
+                    SPHSTK_REQUIRE(order_[j] == order_[i] && rad_[j] == rad_[i] && pos_[3 * j] == pos_[3 * i] &&
+                                       pos_[3 * j + 1] == pos_[3 * i + 1] && pos_[3 * j + 2] == pos_[3 * i + 2] &&
+                                       std::equal(&quat_[4 * j], &quat_[4 * j + 4], &quat_[4 * i]),
+                                   "copies of the suspensions must be identical");
+                }
+            const std::vector<int64_t> so0(susp_sph_off_.begin(), susp_sph_off_.begin() + b0 + 1),
+                po0(pt_off_.begin(), pt_off_.begin() + s0 + 1);
+            const std::vector<Runs> r0(runs_fine_.begin(), runs_fine_.begin() + s0);
+            build_far_lists(lists_fine_, b0, so0, po0, po0, r0, 1, split, kWantItems);
+            if (lists_fine_.split > 1) lists_fine_.part.alloc(size_t(lists_fine_.split) * 3 * npts_);
+            srcm_.alloc(std::max<int64_t>(4 * (ncopy_ + 1) * pt_off_[s0], 8));
+        } else {
+            build_far_lists(lists_fine_, nsusp_, susp_sph_off_, pt_off_, pt_off_, runs_fine_, tpl, split, kWantItems,
+                            far_mfma_ ? pos_.data() : nullptr);
+        }
     }
     ls32_.tpl = ls64_.tpl = tpl;
 
@@ -332,7 +357,8 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     coef_.alloc(std::max<int64_t>(coef_off_[nsph_], 1));
     rec_.alloc(std::max<int64_t>(rec_off_[nsph_], 1));
     tmp_.alloc(std::max<int64_t>(3 * npts_, 1));
-    if ((flags_ & 4) && std::getenv("SPHSTK_FAR_TPL") == nullptr && std::getenv("SPHSTK_FAR_SPLIT") == nullptr) {
+    if ((flags_ & 4) && ncopy_ == 1 && std::getenv("SPHSTK_FAR_TPL") == nullptr &&
+        std::getenv("SPHSTK_FAR_SPLIT") == nullptr) {
         double budget = kTuneBudget;
         autotune_fine(budget);
     }
@@ -1031,6 +1057,35 @@ void Operator::far_levels(LevelSet &S, int kind, double coef, const double *in, 
         launch_resample(geom_, L->up, L->d_off_by_sph.p, d_pt_off_.p, L->tv.p, out, 1, active, stream_, L->max_pts);
 }
 
+// FP64 far field of ncopy identical copies of a set of suspensions (several
+// right-hand sides on one geometry), with the first copy's work lists: the
+// traction sum once per pair for all copies (k_far_trac_multi_f64, 2-4
+// copies); the single layer, or more copies, one copy at a time on the first
+// copy's geometry.
+void Operator::far_copies(int kind, double coef, const double *in, double *out, const int *active, hipEvent_t ev_a) {
+    FarLists &L = lists_fine_;
+    const int b0 = nsusp_ / ncopy_;
+    const int64_t n0 = npts_ / ncopy_, col = 3 * n0;
+    const bool split = L.split > 1;
+    if (ev_a) SPHSTK_HIP(hipEventRecord(ev_a, stream_));
+    if (L.pairs == 0.0) return;
+    if (kind == 0 && ncopy_ <= 4) {
+        launch_far_trac_multi(ncopy_, n0, d_coords_.p, d_norms_.p, d_w_.p, in, col, srcm_.p, L.ngroups, L.order.p,
+                              L.groups.p, L.iv.p, active, b0, split ? L.part.p : out, coef, split ? 0 : 1, 3 * npts_,
+                              stream_);
+        if (split) launch_slot_reduce(out, L.part.p, L.split, 3 * npts_, chunks_.p, (int)chunks_.n, active, stream_);
+        return;
+    }
+    const int nch0 = seg_chunk_off_host_[b0];
+    for (int c = 0; c < ncopy_; c++) {
+        const int *act = active ? active + c * b0 : nullptr;
+        launch_pack_src_f64(n0, d_coords_.p, d_w_.p, in + c * col, src64_.p, stream_);
+        launch_far(kind, 0, L.tpl, src64_.p, d_coords_.p, d_norms_.p, nullptr, L.ntrg, L.ngroups, L.order.p,
+                   L.groups.p, L.iv.p, act, split ? L.part.p : out + c * col, coef, split ? 0 : 1, stream_);
+        if (split) launch_slot_reduce(out + c * col, L.part.p, L.split, 3 * L.ntrg, chunks_.p, nch0, act, stream_);
+    }
+}
+
 // The FP64 multi-level far field is used when its pair evaluations (fine mid
 // range plus all levels) are at most 3/4 of the direct sum's; below that the
 // resampling and the extra launches eat the saving (many small suspensions).
@@ -1061,7 +1116,10 @@ void Operator::far(int kind, int prec, double coef, const double *in, double *ou
         ev.kind = kind;
         ev.prec = prec;
     }
-    if (prec == 0) {
+    if (ncopy_ > 1) {
+        SPHSTK_REQUIRE(prec == 0, "several right-hand sides on one geometry are solved in FP64 only");
+        far_copies(kind, coef, in, out, active, timing ? ev.a : nullptr);
+    } else if (prec == 0) {
         launch_pack_src_f64(npts_, d_coords_.p, d_w_.p, in, src64_.p, stream_);
         if (timing) SPHSTK_HIP(hipEventRecord(ev.a, stream_));
         far_run(lists_fine_, kind, 0, src64_.p, d_coords_.p, d_norms_.p, nullptr, chunks_.p, (int)chunks_.n, coef,
@@ -1399,7 +1457,7 @@ void Operator::kernel_stats(int kind, int prec, double *ms, int64_t *launches, d
             *pairs = S->mid.pairs;
             for (auto &L : S->levels) *pairs += L->lists.pairs;
         } else {
-            *pairs = lists_fine_.pairs;
+            *pairs = lists_fine_.pairs * (prec == 0 ? ncopy_ : 1); // copies: per-copy lists, all copies summed
         }
     }
 }
@@ -1414,7 +1472,7 @@ void Operator::reset_stats() {
 // Mobility: SphereSTKMobMat (BIE/SphereSTKMobMat.cpp:6-307) with A = 0.5 I + K' + L (:17)
 
 Mobility::Mobility(int nsusp, const int *counts, const double *pos, const double *radius, const double *quat,
-                   const int *order, double viscosity, const SolverParams &prm, int precision, int flags)
+                   const int *order, double viscosity, const SolverParams &prm, int precision, int flags, int ncopy)
     : mu_(viscosity), prm_(prm), prec_(precision) {
     if (const char *v = std::getenv("SPHSTK_OUTER_ML")) outer_prec_ = std::atoi(v) != 0 ? 2 : 0;
     inner_restart_ = std::max(prm.restart, 60);
@@ -1423,7 +1481,8 @@ Mobility::Mobility(int nsusp, const int *counts, const double *pos, const double
     SPHSTK_REQUIRE(std::isfinite(viscosity) && viscosity > 0, "viscosity must be > 0");
     SPHSTK_REQUIRE(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (mixed)");
     if (const char *v = std::getenv("SPHSTK_VERBOSE")) verbose_ = std::atoi(v) != 0;
-    A_ = std::make_unique<Operator>(nsusp, counts, pos, radius, quat, order, 0.5, 0.0, 1.0, 1.0, flags);
+    SPHSTK_REQUIRE(ncopy == 1 || precision == 0, "several right-hand sides on one geometry: FP64 form only");
+    A_ = std::make_unique<Operator>(nsusp, counts, pos, radius, quat, order, 0.5, 0.0, 1.0, 1.0, flags, ncopy);
     const int64_t n = std::max<int64_t>(A_->ndof(), 1);
     rho_.alloc(n);
     b_.alloc(n);
@@ -1772,6 +1831,11 @@ void Mobility::get_unknown(double *x) const {
     SPHSTK_HIP(hipStreamSynchronize(A_->stream()));
     SPHSTK_HIP(hipMemcpy(x, x_.p, sizeof(double) * A_->ndof(), hipMemcpyDeviceToHost));
 }
+void Mobility::set_unknown_zero() {
+    launch_fill(A_->ndof(), 0.0, x_.p, A_->stream());
+    x_zero_ = true;
+}
+
 void Mobility::set_unknown(const double *x) {
     SPHSTK_HIP(hipStreamSynchronize(A_->stream()));
     x_zero_ = std::all_of(x, x + A_->ndof(), [](double v) { return v == 0.0; });

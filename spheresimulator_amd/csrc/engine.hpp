@@ -127,8 +127,11 @@ struct SegInfo {
 
 class Operator {
   public:
+    // ncopy > 1: the nsusp suspensions are ncopy identical copies of the first
+    // nsusp / ncopy (several right-hand sides on one geometry, FP64 only): the
+    // far field is computed once per pair for all copies.
     Operator(int nsusp, const int *counts, const double *pos, const double *radius, const double *quat,
-             const int *order, double cId, double cSL, double cTrac, double cLOP, int flags);
+             const int *order, double cId, double cSL, double cTrac, double cLOP, int flags, int ncopy = 1);
     ~Operator();
 
     // sizes
@@ -191,6 +194,7 @@ class Operator {
     void far_run(FarLists &L, int kind, int prec, const void *src, const double *tcoords, const double *tnorms,
                  const float *trg32, const Chunk *chunks, int nchunks, double coef, double *out, const int *active);
     void ensure_levels(LevelSet &S);
+    void far_copies(int kind, double coef, const double *in, double *out, const int *active, hipEvent_t ev_a);
     // tile-shape autotuner: times candidate work-list shapes on this problem
     void autotune_fine(double &budget_s);
     void autotune_inner(double &budget_s);
@@ -207,6 +211,8 @@ class Operator {
 
     int nsph_, nsusp_;
     int64_t npts_;
+    int ncopy_ = 1;         // identical copies of one set of suspensions (several right-hand sides)
+    DevBuf<double> srcm_;   // source records of the shared far field of the copies
     double c_[4];
     int flags_;
     int max_order_ = 0;
@@ -261,6 +267,7 @@ class Operator {
     DevBuf<float> src32_, trg32_;
     DevBuf<Chunk> chunks_;
     DevBuf<int> seg_chunk_off_;
+    std::vector<int> seg_chunk_off_host_;
     DevBuf<double> partial_;
 
     // timing
@@ -277,8 +284,12 @@ class Operator {
 class Mobility {
   public:
     Mobility(int nsusp, const int *counts, const double *pos, const double *radius, const double *quat,
-             const int *order, double viscosity, const SolverParams &prm, int precision, int flags);
+             const int *order, double viscosity, const SolverParams &prm, int precision, int flags, int ncopy = 1);
     Operator &op() { return *A_; }
+    int precision() const { return prec_; }
+    const SolverParams &params() const { return prm_; }
+    // the next solve starts from x0 = 0
+    void set_unknown_zero();
     void set_params(const SolverParams &prm, int precision) {
         prm_ = prm;
         prec_ = precision;

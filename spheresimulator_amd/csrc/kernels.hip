@@ -1183,6 +1183,141 @@ void launch_centre_finish(int nsph, int nslot, const double *part, const double 
 }
 
 // ---------------------------------------------------------------------------
+// FP64 traction far field of NC densities on one geometry (NC right-hand
+// sides solved in lock step: copies of one set of suspensions).  The work
+// lists are those of the first copy; per source the record holds
+// (x, y, z, 0, w f_0, 0, w f_1, 0, ...), 4 (NC + 1) doubles, so that d, r^2,
+// r^-5 and d.n are computed once per pair for all NC densities:
+// 15 VALU instructions + one v_rsq_f64 shared, 7 per density.
+template <int NC>
+__global__ void k_pack_src_multi(int64_t n0, const double *__restrict__ coords, const double *__restrict__ w,
+                                 const double *__restrict__ in, int64_t col_stride, double *__restrict__ src) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n0) return;
+    double *o = src + 4 * (NC + 1) * i;
+    const double wi = w[i];
+    o[0] = coords[3 * i];
+    o[1] = coords[3 * i + 1];
+    o[2] = coords[3 * i + 2];
+    o[3] = 0.0;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const double *v = in + c * col_stride + 3 * i;
+        o[4 * (c + 1)] = wi * v[0];
+        o[4 * (c + 1) + 1] = wi * v[1];
+        o[4 * (c + 1) + 2] = wi * v[2];
+        o[4 * (c + 1) + 3] = 0.0;
+    }
+}
+
+template <int NC>
+__global__ __launch_bounds__(64) void k_far_trac_multi_f64(const double *__restrict__ src,
+                                                           const double *__restrict__ coords,
+                                                           const double *__restrict__ norms,
+                                                           const int *__restrict__ gorder,
+                                                           const FarGroup *__restrict__ groups,
+                                                           const FarInterval *__restrict__ iv,
+                                                           const int *__restrict__ active, int nsusp0,
+                                                           double *__restrict__ out, double coef, int accumulate,
+                                                           int64_t slot_stride, int64_t col_stride) {
+    const FarGroup G = groups[gorder[blockIdx.x]];
+    bool any = !active;
+#pragma unroll
+    for (int c = 0; c < NC; c++) any = any || active[c * nsusp0 + G.susp];
+    if (!any) return;
+    const int lane = threadIdx.x;
+    const int64_t t = G.tstart + (lane < G.tcount ? lane : 0);
+    const double x = coords[3 * t], y = coords[3 * t + 1], z = coords[3 * t + 2];
+    const double nx = norms[3 * t], ny = norms[3 * t + 1], nz = norms[3 * t + 2];
+    double ax[NC], ay[NC], az[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) ax[c] = ay[c] = az[c] = 0.0;
+    constexpr int RS = 4 * (NC + 1);
+    auto pair = [&](const double *v, bool incl) {
+        const double dx = x - v[0], dy = y - v[1], dz = z - v[2];
+        const double r2 = fmad(dx, dx, fmad(dy, dy, dz * dz));
+        double wq = trac_w_f64(r2);
+        wq = incl ? wq : 0.0;
+        const double dnw = fmad(dx, nx, fmad(dy, ny, dz * nz)) * wq;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const double *f = v + 4 * (c + 1);
+            const double q = fmad(dx, f[0], fmad(dy, f[1], dz * f[2])) * dnw;
+            ax[c] = fmad(q, dx, ax[c]);
+            ay[c] = fmad(q, dy, ay[c]);
+            az[c] = fmad(q, dz, az[c]);
+        }
+    };
+    for (int64_t k = G.iv_begin; k < G.iv_end; k++) {
+        const int64_t b = iv[k].begin, e = iv[k].end;
+        const bool incl = ((iv[k].mask[0] >> lane) & 1ull) != 0;
+        int64_t s = b;
+        // two sources per step: their wave-uniform (scalar) loads are issued
+        // before the arithmetic of either
+        for (; s + 2 <= e; s += 2) {
+            double v[2][RS];
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+#pragma unroll
+                for (int j = 0; j < RS; j++) v[u][j] = src[RS * (s + u) + j];
+            pair(v[0], incl);
+            pair(v[1], incl);
+        }
+        if (s < e) {
+            double v[RS];
+#pragma unroll
+            for (int j = 0; j < RS; j++) v[j] = src[RS * s + j];
+            pair(v, incl);
+        }
+    }
+    if (lane >= G.tcount) return;
+    const double cf = kTracW * coef;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        if (active && !active[c * nsusp0 + G.susp]) continue;
+        double *o = out + (accumulate ? 0 : G.slot * slot_stride) + c * col_stride + 3 * (G.tstart + lane);
+        if (accumulate) {
+            o[0] += cf * ax[c];
+            o[1] += cf * ay[c];
+            o[2] += cf * az[c];
+        } else {
+            o[0] = cf * ax[c];
+            o[1] = cf * ay[c];
+            o[2] = cf * az[c];
+        }
+    }
+}
+
+void launch_far_trac_multi(int ncol, int64_t n0, const double *coords, const double *norms, const double *w,
+                           const double *in, int64_t col_stride, double *src, int ngroups, const int *go,
+                           const FarGroup *gr, const FarInterval *iv, const int *active, int nsusp0, double *out,
+                           double coef, int acc, int64_t slot_stride, hipStream_t s) {
+    const dim3 pb((unsigned)((n0 + 255) / 256)), pt(256);
+    switch (ncol) {
+    case 2:
+        if (n0 > 0) k_pack_src_multi<2><<<pb, pt, 0, s>>>(n0, coords, w, in, col_stride, src);
+        if (ngroups > 0)
+            k_far_trac_multi_f64<2><<<dim3(ngroups), dim3(64), 0, s>>>(src, coords, norms, go, gr, iv, active, nsusp0,
+                                                                      out, coef, acc, slot_stride, col_stride);
+        break;
+    case 3:
+        if (n0 > 0) k_pack_src_multi<3><<<pb, pt, 0, s>>>(n0, coords, w, in, col_stride, src);
+        if (ngroups > 0)
+            k_far_trac_multi_f64<3><<<dim3(ngroups), dim3(64), 0, s>>>(src, coords, norms, go, gr, iv, active, nsusp0,
+                                                                      out, coef, acc, slot_stride, col_stride);
+        break;
+    case 4:
+        if (n0 > 0) k_pack_src_multi<4><<<pb, pt, 0, s>>>(n0, coords, w, in, col_stride, src);
+        if (ngroups > 0)
+            k_far_trac_multi_f64<4><<<dim3(ngroups), dim3(64), 0, s>>>(src, coords, norms, go, gr, iv, active, nsusp0,
+                                                                      out, coef, acc, slot_stride, col_stride);
+        break;
+    default:
+        throw Error(-1, "shared far field supports 2 to 4 right-hand sides");
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Traction far field of the rigid-body densities rho0 = F/(4 pi R^2) +
 // 3/(8 pi R^4) T x (R n) (BIE/SphereSTKMobMat.cpp:174-192) of distant spheres,
 // in closed form at the sphere centres.  Outside its sphere, rho0 of sphere S
