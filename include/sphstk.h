@@ -209,6 +209,32 @@ int sphstk_mobility_get_density(const sphstk_mobility *mob, double *rho);
 int sphstk_mobility_get_unknown(const sphstk_mobility *mob, double *x);
 int sphstk_mobility_set_unknown(sphstk_mobility *mob, const double *x);
 
+/* ---- The steps beside the solve (host arrays; the work runs on the GPU) ---- */
+
+/* Brownian velocities, replacing SphereSystem::getVelocityBrown
+ * (SRC/SphereSystem.cpp:323-360): per sphere i of radius R,
+ *   vel[6i..6i+2] = sqrt(1/(6 pi R mu)) sqrt(2 kBT/dt) scale z_0..2,
+ *   vel[6i+3..6i+5] = sqrt(1/(8 pi R^3 mu)) sqrt(2 kBT/dt) scale z_3..5,
+ * with standard normals z from Philox4x32-10 blocks (counter (i, c, step_lo,
+ * step_hi), c = 0..2, key (seed_lo, seed_hi)) through Box-Muller: the same
+ * (seed, step) gives the same numbers on any device and launch.  The reference
+ * draws from a per-thread TRNG pool instead, so its streams are not reproduced. */
+int sphstk_brownian_velocity(int n, const double *radius, double viscosity, double kBT, double dt, double scale,
+                             uint64_t seed, uint64_t step, double *vel);
+
+/* Collision pairs, replacing the near search of SphereSystem::resolveCollision
+ * (SRC/SphereSystem.cpp:414-431) and CollisionSphere::collide
+ * (Sphere/CollisionSphere.hpp:51-90): every pair (I, J) with gid[I] < gid[J]
+ * (gid NULL: the index) and sep = |x_J - x_I| - Rc_I - Rc_J < colbuf * Rc_I,
+ * ordered by I, then J.  Per pair: idx_i, idx_j (sphere indices), phi0 = sep,
+ * norm_i = -(x_J - x_I)/|x_J - x_I| (3 doubles; normJ = -normI, the initial
+ * gamma max(-sep, 0) and the stress |r| normI normI^T follow from these).
+ * *npairs receives the count; the arrays (any may be NULL) are written only
+ * when it is <= capacity, so capacity 0 asks for the count alone.  colbuf is
+ * the collision buffer of SimToolbox's GeoCommon.h, an argument here. */
+int sphstk_collision_pairs(int n, const double *pos, const double *radius_col, const int *gid, double colbuf,
+                           int64_t capacity, int64_t *npairs, int *idx_i, int *idx_j, double *phi0, double *norm_i);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1020,3 +1020,100 @@ int orc_mobility_solve(const orc_op *A, double viscosity, const double *force, d
     free(x);
     return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* The steps beside the solve: Brownian velocities and collision pairs.      */
+
+/* Philox4x32-10 (Salmon, Moraes, Dror and Shaw, "Parallel random numbers: as
+ * easy as 1, 2, 3", SC'11): ten rounds of two 32x32->64 multiplies with
+ * multipliers 0xD2511F53 and 0xCD9E8D57, the key bumped by the Weyl constants
+ * 0x9E3779B9 and 0xBB67AE85 between rounds. */
+void orc_philox4x32_10(const unsigned *ctr, const unsigned *key, unsigned *out) {
+    unsigned c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const unsigned long long p0 = (unsigned long long)0xD2511F53u * c0, p1 = (unsigned long long)0xCD9E8D57u * c2;
+        const unsigned hi0 = (unsigned)(p0 >> 32), lo0 = (unsigned)p0, hi1 = (unsigned)(p1 >> 32), lo1 = (unsigned)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+/* six standard normals of sphere i at `step`: three Philox blocks with counter
+ * (i, c, step_lo, step_hi), key (seed_lo, seed_hi); each block gives two
+ * 53-bit uniforms in (0,1) and one Box-Muller pair */
+static void brown_normals(unsigned long long seed, unsigned long long step, int i, double z[6]) {
+    const unsigned key[2] = {(unsigned)seed, (unsigned)(seed >> 32)};
+    for (int c = 0; c < 3; c++) {
+        const unsigned ctr[4] = {(unsigned)i, (unsigned)c, (unsigned)step, (unsigned)(step >> 32)};
+        unsigned r[4];
+        orc_philox4x32_10(ctr, key, r);
+        const double ua = ((double)(((unsigned long long)r[0] << 21) | (r[1] >> 11)) + 0.5) * 0x1p-53;
+        const double ub = ((double)(((unsigned long long)r[2] << 21) | (r[3] >> 11)) + 0.5) * 0x1p-53;
+        const double m = sqrt(-2.0 * log(ua)), th = 2.0 * PI * ub;
+        z[2 * c] = m * cos(th);
+        z[2 * c + 1] = m * sin(th);
+    }
+}
+
+/* SphereSystem::getVelocityBrown (SRC/SphereSystem.cpp:323-360): per sphere
+ * V = sqrt(1/(6 pi R mu)) sqrt(2 kBT/dt) scale N(0,1)^3 and
+ * W = sqrt(1/(8 pi R^3 mu)) sqrt(2 kBT/dt) scale N(0,1)^3.  The reference draws
+ * from a per-thread TRNG pool; the normals here come from brown_normals. */
+void orc_brownian_velocity(int n, const double *radius, double mu, double kBT, double dt, double scale,
+                           unsigned long long seed, unsigned long long step, double *vel) {
+    const double fack = sqrt(2.0 * kBT / dt) * scale;
+    for (int i = 0; i < n; i++) {
+        const double R = radius[i];
+        const double dx = sqrt(1.0 / (6.0 * PI * R * mu)) * fack, dr = sqrt(1.0 / (8.0 * PI * R * R * R * mu)) * fack;
+        double z[6];
+        brown_normals(seed, step, i, z);
+        for (int k = 0; k < 3; k++) {
+            vel[6 * i + k] = dx * z[k];
+            vel[6 * i + 3 + k] = dr * z[3 + k];
+        }
+    }
+}
+
+/* Collision pairs (CollisionSphere::collide, Sphere/CollisionSphere.hpp:51-90):
+ * sphere a is the I side of the pair with b when gid[a] < gid[b] and the
+ * separation sep = |x_b - x_a| - Rc_a - Rc_b is below colbuf * Rc_a; the block
+ * holds phi0 = sep and normI = -(x_b - x_a)/|x_b - x_a|.  Brute force over all
+ * pairs, in the order (a ascending, then b ascending).  Returns the number of
+ * pairs; arrays (may be NULL) are written up to `cap` entries. */
+long orc_collision_pairs(int n, const double *pos, const double *rcol, const int *gid, double colbuf, long cap,
+                         int *idx_i, int *idx_j, double *phi0, double *norm_i) {
+    long np = 0;
+    for (int a = 0; a < n; a++)
+        for (int b = 0; b < n; b++) {
+            const int ga = gid ? gid[a] : a, gb = gid ? gid[b] : b;
+            if (ga >= gb) continue;
+            const double dx = pos[3 * b] - pos[3 * a], dy = pos[3 * b + 1] - pos[3 * a + 1],
+                         dz = pos[3 * b + 2] - pos[3 * a + 2];
+            const double r = sqrt(fma(dx, dx, fma(dy, dy, dz * dz)));
+            const double sep = (r - rcol[a]) - rcol[b];
+            if (!(sep < colbuf * rcol[a])) continue;
+            if (np < cap) {
+                const double s = -1.0 / r;
+                if (idx_i) idx_i[np] = a;
+                if (idx_j) idx_j[np] = b;
+                if (phi0) phi0[np] = sep;
+                if (norm_i) {
+                    norm_i[3 * np] = dx * s;
+                    norm_i[3 * np + 1] = dy * s;
+                    norm_i[3 * np + 2] = dz * s;
+                }
+            }
+            np++;
+        }
+    return np;
+}

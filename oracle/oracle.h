@@ -18,6 +18,9 @@
  *                        (BIE/SphereSTKSHOperator.cpp:547)
  *   mobility solve       BIE/SphereSTKMobMat.cpp:146-286, GMRES parameters
  *                        mobilitySolver.yaml:3-16
+ *   Brownian velocities  SRC/SphereSystem.cpp:323-360 (normals from Philox4x32-10
+ *                        + Box-Muller in place of the reference's TRNG pool)
+ *   collision pairs      Sphere/CollisionSphere.hpp:51-90 (brute force)
  *
  * The spherical-harmonic arithmetic the reference takes from SCTL (an un-vendored,
  * unpinned dependency) is re-derived: vector SH of degree <= p in the basis
@@ -110,6 +113,15 @@ int orc_mobility_solve(const orc_op *A, double viscosity, const double *force, d
                        orc_gmres_params prm, orc_solve_info *info);
 
 int orc_num_threads(void);
+
+/* Philox4x32-10 block: out = philox(ctr[4], key[2]). */
+void orc_philox4x32_10(const unsigned *ctr, const unsigned *key, unsigned *out);
+/* Brownian velocities, 6 per sphere (SRC/SphereSystem.cpp:323-360). */
+void orc_brownian_velocity(int n, const double *radius, double mu, double kBT, double dt, double scale,
+                           unsigned long long seed, unsigned long long step, double *vel);
+/* Collision pairs (Sphere/CollisionSphere.hpp:51-90), brute force; returns the count. */
+long orc_collision_pairs(int n, const double *pos, const double *rcol, const int *gid, double colbuf, long cap,
+                         int *idx_i, int *idx_j, double *phi0, double *norm_i);
 
 #ifdef __cplusplus
 }

@@ -69,6 +69,12 @@ def lib():
         L.orc_mobility_solve.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp, _dp, _dp, GmresParams,
                                          ctypes.POINTER(SolveInfo)]
         L.orc_num_threads.restype = ctypes.c_int
+        _up = ctypes.POINTER(ctypes.c_uint)
+        L.orc_philox4x32_10.argtypes = [_up, _up, _up]
+        L.orc_brownian_velocity.argtypes = [ctypes.c_int, _dp] + [ctypes.c_double] * 4 + [ctypes.c_ulonglong] * 2 + [_dp]
+        L.orc_collision_pairs.argtypes = [ctypes.c_int, _dp, _dp, _ip, ctypes.c_double, ctypes.c_long, _ip, _ip, _dp,
+                                          _dp]
+        L.orc_collision_pairs.restype = ctypes.c_long
         _LIB = L
     return _LIB
 
@@ -244,3 +250,40 @@ class MobilityOperator(Operator):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def philox4x32_10(ctr, key):
+    """One Philox4x32-10 block: four 32-bit words from counter (4 words) and key (2 words)."""
+    c = np.ascontiguousarray(np.asarray(ctr, dtype=np.uint32))
+    k = np.ascontiguousarray(np.asarray(key, dtype=np.uint32))
+    o = np.zeros(4, dtype=np.uint32)
+    up = ctypes.POINTER(ctypes.c_uint)
+    lib().orc_philox4x32_10(c.ctypes.data_as(up), k.ctypes.data_as(up), o.ctypes.data_as(up))
+    return o
+
+
+def brownian_velocity(radius, dt, kBT, viscosity=1.0, scale=1.0, seed=0, step=0):
+    r = _c(np.asarray(radius, dtype=np.float64).ravel())
+    v = np.zeros(6 * r.size)
+    lib().orc_brownian_velocity(r.size, _d(r), viscosity, kBT, dt, scale, seed, step, _d(v))
+    return v
+
+
+def collision_pairs(pos, radius_col, colbuf, gid=None):
+    """Brute-force collision pairs: (idx_i, idx_j, phi0, norm_i[m, 3]) ordered by I then J."""
+    p = _c(np.asarray(pos, dtype=np.float64).reshape(-1, 3))
+    rc = _c(np.asarray(radius_col, dtype=np.float64).ravel())
+    n = rc.size
+    ip = ctypes.POINTER(ctypes.c_int)
+    g = None if gid is None else _c(np.asarray(gid, dtype=np.int32).ravel(), np.int32)
+    gp = g.ctypes.data_as(ip) if g is not None else None
+    m = lib().orc_collision_pairs(n, _d(p), _d(rc), gp, colbuf, 0, None, None, None, None)
+    ii = np.zeros(m, dtype=np.int32)
+    jj = np.zeros(m, dtype=np.int32)
+    phi0 = np.zeros(m)
+    nrm = np.zeros(3 * m)
+    if m:
+        lib().orc_collision_pairs(n, _d(p), _d(rc), gp, colbuf, m, ii.ctypes.data_as(ip), jj.ctypes.data_as(ip),
+                                  _d(phi0), _d(nrm))
+    return ii, jj, phi0, nrm.reshape(-1, 3)
+

@@ -3,6 +3,7 @@
 // kept per calling thread for sphstk_last_error().
 #include "../../include/sphstk.h"
 #include "engine.hpp"
+#include "steps.hpp"
 
 #include <cstring>
 #include <new>
@@ -445,6 +446,19 @@ int sphstk_mobility_set_unknown(sphstk_mobility *mob, const double *x) {
     return guard([&] {
         SPHSTK_REQUIRE(mob && x, "invalid arguments");
         mob->mob->set_unknown(x);
+    });
+}
+
+int sphstk_brownian_velocity(int n, const double *radius, double viscosity, double kBT, double dt, double scale,
+                             uint64_t seed, uint64_t step, double *vel) {
+    return guard([&] { brownian_velocity(n, radius, viscosity, kBT, dt, scale, seed, step, vel); });
+}
+
+int sphstk_collision_pairs(int n, const double *pos, const double *radius_col, const int *gid, double colbuf,
+                           int64_t capacity, int64_t *npairs, int *idx_i, int *idx_j, double *phi0, double *norm_i) {
+    return guard([&] {
+        SPHSTK_REQUIRE(npairs, "npairs must not be NULL");
+        *npairs = collision_pairs(n, pos, radius_col, gid, colbuf, capacity, idx_i, idx_j, phi0, norm_i);
     });
 }
 

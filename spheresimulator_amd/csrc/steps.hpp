@@ -1,0 +1,20 @@
+// The steps beside the mobility solve: Brownian velocities and collision pairs
+// (host arrays in and out; the work runs on the current HIP device).
+#pragma once
+#include "common.hpp"
+
+#include <cstdint>
+
+namespace sphstk {
+
+// vel[6i..6i+5] = (sqrt(1/(6 pi R mu)) N^3, sqrt(1/(8 pi R^3 mu)) N^3) * sqrt(2 kBT/dt) * scale
+// with standard normals from Philox4x32-10 (counter (i, c, step), key seed) and Box-Muller
+void brownian_velocity(int n, const double *radius, double mu, double kBT, double dt, double scale,
+                       unsigned long long seed, unsigned long long step, double *vel);
+
+// pairs (I, J) with gid[I] < gid[J] and |x_J - x_I| - Rc_I - Rc_J < colbuf Rc_I, ordered by I then J;
+// returns the count; the arrays (any may be NULL) are written only when the count <= capacity
+int64_t collision_pairs(int n, const double *pos, const double *rcol, const int *gid, double colbuf,
+                        int64_t capacity, int *ii, int *jj, double *phi0, double *norm);
+
+} // namespace sphstk

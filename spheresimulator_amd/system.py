@@ -1,9 +1,16 @@
 """The steps on either side of the mobility solve in the reference's time loop:
 the XYZ position files (SphereSystem::readXYZ / writeXYZ,
-SRC/SphereSystem.cpp:129-179) and the explicit Euler step of positions and
-orientations (Sphere::stepEuler, Sphere/Sphere.cpp:367-373).  Host-side;
-orientations are unit quaternions (w, x, y, z) as at the C ABI."""
+SRC/SphereSystem.cpp:129-179), the explicit Euler step of positions and
+orientations (Sphere::stepEuler, Sphere/Sphere.cpp:367-373) and the density
+VTU files (Shexp::writeVTU) on the host; Brownian velocities
+(SRC/SphereSystem.cpp:323-360) and collision pairs
+(Sphere/CollisionSphere.hpp:51-90) on the GPU through the C ABI.
+Orientations are unit quaternions (w, x, y, z) as at the C ABI."""
+import ctypes
+
 import numpy as np
+
+from ._lib import check, load
 
 
 def read_xyz(path):
@@ -160,3 +167,56 @@ def write_vtu(path, op, pos, radius, quat, order, density, name="stkmob"):
             _array(f, "types", typ, 1, "UInt8")
             f.write("</Cells>\n</Piece>\n")
         f.write("</UnstructuredGrid>\n</VTKFile>\n")
+
+
+def brownian_velocity(radius, dt, kBT, viscosity=1.0, scale=1.0, seed=0, step=0):
+    """SphereSystem::getVelocityBrown (SRC/SphereSystem.cpp:323-360) on the GPU:
+    [V, Omega] per sphere (6N) with V ~ sqrt(2 kBT/dt/(6 pi mu R)) scale N(0, I) and
+    Omega ~ sqrt(2 kBT/dt/(8 pi mu R^3)) scale N(0, I).  The normals come from
+    Philox4x32-10 with counter (sphere, block, step) and key seed, so (seed, step)
+    fixes them."""
+    radius = np.ascontiguousarray(np.asarray(radius, dtype=np.float64).ravel())
+    vel = np.zeros(6 * radius.size)
+    check(load().sphstk_brownian_velocity(radius.size, radius.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                          viscosity, kBT, dt, scale, seed, step,
+                                          vel.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return vel
+
+
+def collision_pairs(pos, radius_col, colbuf, gid=None):
+    """Collision blocks of SphereSystem::resolveCollision / CollisionSphere::collide
+    (SRC/SphereSystem.cpp:414-431, Sphere/CollisionSphere.hpp:51-90) on the GPU:
+    pairs (I, J) with gid[I] < gid[J] and sep = |x_J - x_I| - Rc_I - Rc_J < colbuf Rc_I,
+    ordered by I then J.  Returns a dict of idx_i, idx_j, phi0 (= sep), norm_i
+    (= -(x_J - x_I)/|x_J - x_I|), norm_j = -norm_i, gamma (max(-sep, 0)) and the
+    stress |x_J - x_I| norm_i norm_i^T per pair."""
+    pos = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).reshape(-1, 3))
+    rc = np.ascontiguousarray(np.asarray(radius_col, dtype=np.float64).ravel())
+    n = len(rc)
+    if len(pos) != n:
+        raise ValueError("pos and radius_col must describe the same spheres")
+    g = None if gid is None else np.ascontiguousarray(np.asarray(gid, dtype=np.int32).ravel())
+    if g is not None and len(g) != n:
+        raise ValueError("gid must have one entry per sphere")
+    L = load()
+    dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    gp = g.ctypes.data_as(ip) if g is not None else None
+    cap = 6 * n
+    while True:
+        ii = np.zeros(cap, dtype=np.int32)
+        jj = np.zeros(cap, dtype=np.int32)
+        phi0 = np.zeros(cap)
+        nrm = np.zeros(3 * cap)
+        cnt = ctypes.c_int64()
+        check(L.sphstk_collision_pairs(n, pos.ctypes.data_as(dp), rc.ctypes.data_as(dp), gp, colbuf, cap,
+                                       ctypes.byref(cnt), ii.ctypes.data_as(ip), jj.ctypes.data_as(ip),
+                                       phi0.ctypes.data_as(dp), nrm.ctypes.data_as(dp)))
+        if cnt.value <= cap:
+            break
+        cap = cnt.value
+    m = cnt.value
+    ii, jj, phi0, nrm = ii[:m], jj[:m], phi0[:m], nrm[:3 * m].reshape(-1, 3)
+    r = np.linalg.norm(pos[jj] - pos[ii], axis=1)
+    return dict(idx_i=ii, idx_j=jj, phi0=phi0, norm_i=nrm, norm_j=-nrm, gamma=np.maximum(-phi0, 0.0),
+                stress=r[:, None, None] * nrm[:, :, None] * nrm[:, None, :])
+

@@ -1,0 +1,101 @@
+"""GPU parity of the steps beside the solve, through the C ABI, against the
+oracle on identical inputs: Brownian velocities (SRC/SphereSystem.cpp:323-360)
+to 1e-14 of their scale (the Philox words are integer-exact; log, sin and cos
+differ from the host's by an ulp or so) and collision pairs
+(Sphere/CollisionSphere.hpp:51-90) with the same pairs in the same order and
+phi0 and normals to 4e-16 (the same FP64 operations in the same order)."""
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+import spheresimulator_amd as S
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print("[case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("n,seed,step", [(1, 0, 0), (1000, 7, 3), (100003, 2 ** 40 + 5, 2 ** 33 + 1)])
+def test_brownian_matches_oracle(n, seed, step):
+    say("brownian n=%d seed=%d step=%d" % (n, seed, step))
+    rad = np.random.default_rng(n).uniform(0.5, 1.5, n)
+    args = dict(dt=0.02, kBT=0.0041, viscosity=0.9, scale=1.1, seed=seed, step=step)
+    v = S.system.brownian_velocity(rad, **args)
+    w = O.brownian_velocity(rad, **args)
+    f = np.sqrt(2 * 0.0041 / 0.02) * 1.1
+    sc = np.repeat(np.stack([np.sqrt(1 / (6 * np.pi * 0.9 * rad)), np.sqrt(1 / (8 * np.pi * 0.9 * rad ** 3))], 1),
+                   3, axis=1).ravel() * f
+    assert np.abs(v - w).max() <= 1e-14 * sc.max()
+    assert np.abs((v - w) / sc).max() < 1e-13
+
+
+def test_brownian_empty_and_errors():
+    say("brownian empty and invalid")
+    assert S.system.brownian_velocity([], dt=0.1, kBT=1.0).size == 0
+    with pytest.raises(S.SphstkError):
+        S.system.brownian_velocity([1.0, -1.0], dt=0.1, kBT=1.0)
+    with pytest.raises(S.SphstkError):
+        S.system.brownian_velocity([1.0], dt=0.0, kBT=1.0)
+
+
+def _check_pairs(pos, rc, cb, gid=None):
+    got = S.system.collision_pairs(pos, rc, cb, gid=gid)
+    ii, jj, phi0, nrm = O.collision_pairs(pos, rc, cb, gid=gid)
+    assert np.array_equal(got["idx_i"], ii) and np.array_equal(got["idx_j"], jj)
+    assert np.allclose(got["phi0"], phi0, rtol=0, atol=4e-16 * max(1.0, np.abs(phi0).max(initial=0)))
+    assert np.allclose(got["norm_i"], nrm, rtol=0, atol=4e-16)
+    return got
+
+
+def test_collision_pairs_random_polydisperse():
+    say("collision pairs 3000 polydisperse spheres, permuted gid")
+    rng = np.random.default_rng(11)
+    n = 3000
+    pos = rng.uniform(0, 30, (n, 3))
+    rc = rng.uniform(0.3, 1.2, n)
+    got = _check_pairs(pos, rc, 0.3, gid=rng.permutation(n))
+    assert len(got["idx_i"]) > n
+    assert np.array_equal(got["gamma"], np.maximum(-got["phi0"], 0))
+
+
+def test_collision_pairs_lattice_sparse_and_edge_cases():
+    say("collision pairs lattice, sparse clusters, n<2, no pairs")
+    k, d = 8, 2.25
+    g = np.arange(k)
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) * d
+    got = _check_pairs(pos, np.ones(len(pos)), 0.3)
+    assert len(got["idx_i"]) == 3 * k * k * (k - 1)
+    # two tight clusters 1e6 apart: the cell edge grows, the pairs stay exact
+    rng = np.random.default_rng(5)
+    a = rng.uniform(0, 6, (200, 3))
+    pos = np.concatenate([a, a[:100] + 1e6])
+    _check_pairs(pos, np.full(300, 0.5), 0.3)
+    assert len(S.system.collision_pairs(np.zeros((1, 3)), [1.0], 0.3)["idx_i"]) == 0
+    assert len(S.system.collision_pairs(np.zeros((0, 3)), [], 0.3)["idx_i"]) == 0
+    far = np.array([[0.0, 0, 0], [10.0, 0, 0]])
+    assert len(S.system.collision_pairs(far, [1.0, 1.0], 0.3)["idx_i"]) == 0
+    # a capacity below the count returns the count and writes nothing (the binding then retries)
+    import ctypes
+    L = S.load()
+    pos = np.ascontiguousarray(np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) * d, dtype=float)
+    rc = np.ones(len(pos))
+    cnt = ctypes.c_int64()
+    dp = ctypes.POINTER(ctypes.c_double)
+    ii = np.full(4, -7, dtype=np.int32)
+    rc_ = L.sphstk_collision_pairs(len(rc), pos.ctypes.data_as(dp), rc.ctypes.data_as(dp), None, 0.3, 4,
+                                   ctypes.byref(cnt), ii.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), None, None,
+                                   None)
+    assert rc_ == 0 and cnt.value == 3 * k * k * (k - 1) and (ii == -7).all()
+
+
+def test_collision_pairs_cfg5_packing():
+    """The 16384-sphere clustered packing (log-normal radii, vf 0.2): the
+    collision radius = the hydrodynamic one."""
+    say("collision pairs 16384-sphere clustered packing")
+    c = S.configs.config5()
+    _check_pairs(c["pos"], c["radius"], 0.3)
