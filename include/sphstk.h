@@ -235,6 +235,33 @@ int sphstk_brownian_velocity(int n, const double *radius, double viscosity, doub
 int sphstk_collision_pairs(int n, const double *pos, const double *radius_col, const int *gid, double colbuf,
                            int64_t capacity, int64_t *npairs, int *idx_i, int *idx_j, double *phi0, double *norm_i);
 
+/* The collision LCP around the mobility solve, replacing SimToolbox's
+ * CollisionSolver as called by SphereSystem::resolveCollision
+ * (SRC/SphereSystem.cpp:445-474; the solver is not in the reference tree):
+ *   gamma >= 0, phi = phi0 + buffer + dt D^T (vel_known + M D gamma) >= 0, gamma . phi = 0,
+ * with D gamma adding gamma norm_i to sphere idx_i's force and -gamma norm_i to
+ * idx_j's (contact on the line of centres, no torque), solved by
+ * Barzilai-Borwein projected gradient descent to max_k |min(gamma_k, phi_k)| <= tol
+ * (runConfig "LCP tol", 2000 iterations in the reference).  M is one full
+ * mobility solve of `mob` per iteration, warm-started from the previous one
+ * (SphereSTKMobMat::apply inside the LCP), or, with mob NULL, the free-drain
+ * diagonal 1/(6 pi mu R), 1/(8 pi mu R^3) of SphereSystem::getMobOperator
+ * without hydrodynamics (SRC/SphereSystem.cpp:237-268; radius and viscosity
+ * are read only then).  gamma: initial guess in (NULL entries not allowed;
+ * negative values are clipped), solution out.  force_col = D gamma and
+ * vel_col = M D gamma (6N each; the reference's getForceCol / getVelocityCol,
+ * vel = vel_col + vel_known at SRC/SphereSystem.cpp:472-474). */
+typedef struct {
+    int iters;           /* BBPGD iterations */
+    int mobility_solves; /* applications of M */
+    int converged;       /* residual <= tol */
+    double residual;     /* max_k |min(gamma_k, phi_k)| at return */
+} sphstk_lcp_info;
+int sphstk_collision_lcp(sphstk_mobility *mob, int n, const double *radius, double viscosity, int64_t npairs,
+                         const int *idx_i, const int *idx_j, const double *phi0, const double *norm_i,
+                         const double *vel_known, double dt, double buffer, double tol, int max_iters, double *gamma,
+                         double *force_col, double *vel_col, sphstk_lcp_info *info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -287,3 +287,91 @@ def collision_pairs(pos, radius_col, colbuf, gid=None):
                                   _d(phi0), _d(nrm))
     return ii, jj, phi0, nrm.reshape(-1, 3)
 
+
+
+def free_drain(radius, mu):
+    """Diagonal mobility without hydrodynamic interaction (SRC/SphereSystem.cpp:237-268)."""
+    r = np.asarray(radius, dtype=np.float64).ravel()
+    d = np.repeat(np.stack([1 / (6 * np.pi * r * mu), 1 / (8 * np.pi * r ** 3 * mu)], 1), 3, axis=1).ravel()
+    return lambda F: d * F
+
+
+def lcp_bbpgd(mobility, n, ii, jj, phi0, nrm, vel_known, dt, buffer=0.0, tol=1e-5, max_iters=2000, gamma0=None):
+    """Collision LCP by Barzilai-Borwein projected gradient descent (Barzilai and
+    Borwein 1988; Yan et al., J. Comput. Phys. 2019), the solver SimToolbox's
+    CollisionSolver applies around the mobility (SRC/SphereSystem.cpp:445-474):
+    gamma >= 0, phi = phi0 + buffer + dt D^T (v_known + M D gamma) >= 0,
+    gamma . phi = 0; residual max |min(gamma, phi)|.  Small cases only."""
+    ii = np.asarray(ii)
+    jj = np.asarray(jj)
+    nrm = np.asarray(nrm, dtype=np.float64).reshape(-1, 3)
+    m = len(ii)
+
+    def D(g):
+        F = np.zeros((n, 6))
+        np.add.at(F[:, :3], ii, g[:, None] * nrm)
+        np.add.at(F[:, :3], jj, -g[:, None] * nrm)
+        return F.ravel()
+
+    def Dt(v):
+        v = v.reshape(n, 6)[:, :3]
+        return np.einsum("kc,kc->k", nrm, v[ii] - v[jj])
+
+    info = dict(iters=0, mobility_solves=0, converged=0, residual=0.0)
+    if m == 0:
+        info["converged"] = 1
+        return np.zeros(0), np.zeros(6 * n), np.zeros(6 * n), info
+    b = np.asarray(phi0, dtype=np.float64) + buffer + dt * Dt(np.asarray(vel_known, dtype=np.float64))
+
+    def A(x):
+        u = mobility(D(x))
+        info["mobility_solves"] += 1
+        return dt * Dt(u), u
+
+    g = np.maximum(np.zeros(m) if gamma0 is None else np.asarray(gamma0, dtype=np.float64), 0.0)
+    Ag, U = A(g)
+    phi = Ag + b
+    res = np.abs(np.minimum(g, phi)).max()
+    if res > tol:
+        Aphi, _ = A(phi)
+        gAg = phi @ Aphi
+        alpha = (phi @ phi) / gAg if gAg > 0 else 1.0
+        for it in range(1, max_iters + 1):
+            gn = np.maximum(g - alpha * phi, 0.0)
+            Agn, Un = A(gn)
+            phin = Agn + b
+            info["iters"] = it
+            res = np.abs(np.minimum(gn, phin)).max()
+            s, y = gn - g, phin - phi
+            g, phi, U = gn, phin, Un
+            if res <= tol:
+                break
+            ss, sy, yy = s @ s, s @ y, y @ y
+            if ss == 0.0:
+                break
+            a = ss / sy if it % 2 else sy / yy
+            if np.isfinite(a) and a > 0:
+                alpha = a
+    info["residual"] = float(res)
+    info["converged"] = int(res <= tol)
+    return g, D(g), U, info
+
+
+def lcp_enumerate(A, b):
+    """Exact solution of the LCP (A positive definite, hence a P-matrix with one
+    solution; small m) by trying every active set:
+    gamma_S = -A_SS^-1 b_S >= 0 and phi = A gamma + b >= 0 off S."""
+    import itertools
+    m = len(b)
+    best = None
+    for k in range(m + 1):
+        for S in itertools.combinations(range(m), k):
+            S = list(S)
+            g = np.zeros(m)
+            if S:
+                g[S] = np.linalg.solve(A[np.ix_(S, S)], -b[S])
+            phi = A @ g + b
+            if (g >= -1e-12).all() and (phi >= -1e-12).all():
+                if best is None:
+                    best = g
+    return best

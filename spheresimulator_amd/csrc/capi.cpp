@@ -3,6 +3,7 @@
 // kept per calling thread for sphstk_last_error().
 #include "../../include/sphstk.h"
 #include "engine.hpp"
+#include "lcp.hpp"
 #include "steps.hpp"
 
 #include <cstring>
@@ -459,6 +460,35 @@ int sphstk_collision_pairs(int n, const double *pos, const double *radius_col, c
     return guard([&] {
         SPHSTK_REQUIRE(npairs, "npairs must not be NULL");
         *npairs = collision_pairs(n, pos, radius_col, gid, colbuf, capacity, idx_i, idx_j, phi0, norm_i);
+    });
+}
+
+int sphstk_collision_lcp(sphstk_mobility *mob, int n, const double *radius, double viscosity, int64_t npairs,
+                         const int *idx_i, const int *idx_j, const double *phi0, const double *norm_i,
+                         const double *vel_known, double dt, double buffer, double tol, int max_iters, double *gamma,
+                         double *force_col, double *vel_col, sphstk_lcp_info *info) {
+    return guard([&] {
+        MobilityFn M;
+        if (mob) {
+            SPHSTK_REQUIRE(n == mob->mob->op().nsph(), "n must equal the handle's number of spheres");
+            M = [mob](const double *F, double *V) {
+                int outer = 0, a64 = 0, a32 = 0;
+                mob->mob->solve(F, V, mob->last, outer, a64, a32);
+            };
+        } else {
+            SPHSTK_REQUIRE(n == 0 || radius, "radius must not be NULL without a mobility handle");
+            SPHSTK_REQUIRE(std::isfinite(viscosity) && viscosity > 0, "viscosity must be > 0");
+            for (int i = 0; i < n; i++) SPHSTK_REQUIRE(std::isfinite(radius[i]) && radius[i] > 0, "radii must be > 0");
+            M = [n, radius, viscosity](const double *F, double *V) { free_drain_mobility(n, radius, viscosity, F, V); };
+        }
+        const LcpResult r = collision_lcp(M, n, npairs, idx_i, idx_j, phi0, norm_i, vel_known, dt, buffer, tol,
+                                          max_iters, gamma, force_col, vel_col);
+        if (info) {
+            info->iters = r.iters;
+            info->mobility_solves = r.mobility_solves;
+            info->converged = r.converged;
+            info->residual = r.residual;
+        }
     });
 }
 

@@ -10,7 +10,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import check, load
+from ._lib import LcpInfo, check, load
 
 
 def read_xyz(path):
@@ -219,4 +219,39 @@ def collision_pairs(pos, radius_col, colbuf, gid=None):
     r = np.linalg.norm(pos[jj] - pos[ii], axis=1)
     return dict(idx_i=ii, idx_j=jj, phi0=phi0, norm_i=nrm, norm_j=-nrm, gamma=np.maximum(-phi0, 0.0),
                 stress=r[:, None, None] * nrm[:, :, None] * nrm[:, None, :])
+
+
+def collision_lcp(pairs, vel_known, dt, mob=None, radius=None, viscosity=1.0, buffer=0.0, tol=1e-5, max_iters=2000,
+                  gamma0=None):
+    """The collision LCP of SphereSystem::resolveCollision (SRC/SphereSystem.cpp:445-474)
+    by Barzilai-Borwein projected gradient descent in the library: pairs as returned
+    by collision_pairs, vel_known the 6N velocities from the known forces (plus the
+    Brownian term), mob a SphereSTKMobMat (one full mobility solve per iteration) or
+    None for the free-drain mobility of `radius` and `viscosity`.  Returns
+    (gamma, force_col, vel_col, info); the sphere velocities are vel_col + vel_known."""
+    ii = np.ascontiguousarray(pairs["idx_i"], dtype=np.int32)
+    jj = np.ascontiguousarray(pairs["idx_j"], dtype=np.int32)
+    phi0 = np.ascontiguousarray(pairs["phi0"], dtype=np.float64)
+    nrm = np.ascontiguousarray(np.asarray(pairs["norm_i"], dtype=np.float64).reshape(-1, 3))
+    vk = np.ascontiguousarray(np.asarray(vel_known, dtype=np.float64).ravel())
+    n = vk.size // 6
+    if vk.size != 6 * n:
+        raise ValueError("vel_known must hold 6 values per sphere")
+    m = len(ii)
+    g = np.zeros(m) if gamma0 is None else np.ascontiguousarray(np.asarray(gamma0, dtype=np.float64).ravel())
+    if g.size != m:
+        raise ValueError("gamma0 must have one entry per pair")
+    g = g.copy()
+    r = None if radius is None else np.ascontiguousarray(np.asarray(radius, dtype=np.float64).ravel())
+    F = np.zeros(6 * n)
+    U = np.zeros(6 * n)
+    info = LcpInfo()
+    dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    check(load().sphstk_collision_lcp(mob._h if mob is not None else None, n,
+                                      r.ctypes.data_as(dp) if r is not None else None, viscosity, m,
+                                      ii.ctypes.data_as(ip), jj.ctypes.data_as(ip), phi0.ctypes.data_as(dp),
+                                      nrm.ctypes.data_as(dp), vk.ctypes.data_as(dp), dt, buffer, tol, max_iters,
+                                      g.ctypes.data_as(dp), F.ctypes.data_as(dp), U.ctypes.data_as(dp),
+                                      ctypes.byref(info)))
+    return g, F, U, info.as_dict()
 

@@ -99,3 +99,58 @@ def test_collision_pairs_cfg5_packing():
     say("collision pairs 16384-sphere clustered packing")
     c = S.configs.config5()
     _check_pairs(c["pos"], c["radius"], 0.3)
+
+
+def test_collision_lcp_hydro_matches_oracle():
+    """The collision LCP with the hydrodynamic mobility (one GPU mobility solve per
+    BBPGD iteration, SphereSTKMobMat::apply inside the LCP) against the oracle's
+    BBPGD over the oracle's mobility, and against the exact LCP solution from the
+    dense matrix dt D^T M D (every active set tried).  A compressed chain of six
+    spheres with gaps inside the collision buffer."""
+    say("collision LCP, hydrodynamic mobility, 6-sphere chain")
+    rng = np.random.default_rng(4)
+    n = 6
+    pos = np.array([[0, 0, 0], [1.72, 0, 0], [0.86, 1.5, 0], [3.44, 0.1, 0], [5.16, 0, 0.1], [6.9, 0, 0]], float)
+    pos += rng.uniform(-0.02, 0.02, pos.shape)
+    rad = np.full(n, 0.8)
+    quat = np.tile([1.0, 0, 0, 0], (n, 1))
+    order = np.full(n, 6, dtype=np.int32)
+    F = np.zeros((n, 6))
+    F[:, :3] = -(pos - pos.mean(0)) * 10.0  # squeezes the chain
+    F = F.ravel()
+    dt = 0.5
+    mob = S.SphereSTKMobMat(pos, rad, quat, order, params=S.solver_params(tol=1e-10))
+    vk, _ = mob.solveMob(F)
+    pairs = S.system.collision_pairs(pos, rad, 0.3)
+    m = len(pairs["idx_i"])
+    assert m >= 5 and (pairs["phi0"] > 0).all()
+    g, Fc, Uc, info = S.system.collision_lcp(pairs, vk, dt, mob=mob, tol=1e-9)
+    assert info["converged"] == 1 and info["mobility_solves"] >= info["iters"]
+    assert (g > 0).sum() >= 2
+    ref = O.MobilityOperator(pos, rad, quat, order)
+    vko, _ = ref.solve(F, tol=1e-10)
+    assert np.abs(vk - vko).max() < 1e-9 * np.abs(vko).max()
+
+    def Mref(f):
+        return ref.solve(f, tol=1e-10)[0]
+
+    go, Fo, Uo, io = O.lcp_bbpgd(Mref, n, pairs["idx_i"], pairs["idx_j"], pairs["phi0"], pairs["norm_i"], vko, dt,
+                                 tol=1e-9)
+    assert io["converged"] == 1
+    assert np.abs(g - go).max() < 1e-6 * np.abs(go).max()
+    assert np.abs(Uc - Uo).max() < 1e-6 * np.abs(Uo).max()
+    # exact LCP solution of the dense system (positive definite, not symmetric at
+    # this resolution: a P-matrix, so the solution is unique)
+    ii, jj, nrm = pairs["idx_i"], pairs["idx_j"], pairs["norm_i"]
+    A = np.zeros((m, m))
+    for k in range(m):
+        f = np.zeros((n, 6))
+        f[ii[k], :3] += nrm[k]
+        f[jj[k], :3] -= nrm[k]
+        u = Mref(f.ravel()).reshape(n, 6)[:, :3]
+        A[:, k] = dt * np.einsum("kc,kc->k", nrm, u[ii] - u[jj])
+    v = vko.reshape(n, 6)[:, :3]
+    b = pairs["phi0"] + dt * np.einsum("kc,kc->k", nrm, v[ii] - v[jj])
+    assert np.linalg.eigvalsh(0.5 * (A + A.T)).min() > 0
+    want = O.lcp_enumerate(A, b)
+    assert np.abs(g - want).max() < 1e-6 * np.abs(want).max()
