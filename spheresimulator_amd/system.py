@@ -255,3 +255,32 @@ def collision_lcp(pairs, vel_known, dt, mob=None, radius=None, viscosity=1.0, bu
                                       ctypes.byref(info)))
     return g, F, U, info.as_dict()
 
+
+def resolve_collision(pos, radius_col, force_known, dt, colbuf, mob=None, radius=None, viscosity=1.0, kBT=0.0,
+                      scale_brown=0.0, seed=0, step=0, buffer=0.0, tol=1e-5, max_iters=2000, gid=None):
+    """One SphereSystem::resolveCollision (SRC/SphereSystem.cpp:403-478): the known
+    velocity M F_known (a mobility solve of `mob`, or the free drain of `radius`
+    and `viscosity` when mob is None) plus the Brownian term when scale_brown > 0
+    (getVelocityKnown, :280-292), the collision pairs, the LCP, and the sphere
+    velocities vel_col + vel_known (:472-476).  Returns (vel 6N, pairs, gamma, force_col, info)."""
+    F = np.asarray(force_known, dtype=np.float64).ravel()
+    rc = np.asarray(radius_col, dtype=np.float64).ravel()
+    if F.size != 6 * len(rc):
+        raise ValueError("force_known must hold 6 values per sphere")
+    if mob is not None:
+        vk, _ = mob.solveMob(F)
+        rh = rc if radius is None else np.asarray(radius, dtype=np.float64).ravel()
+    else:
+        if radius is None:
+            raise ValueError("radius is needed for the free-drain mobility")
+        rh = np.asarray(radius, dtype=np.float64).ravel()
+        d = np.repeat(np.stack([1 / (6 * np.pi * rh * viscosity), 1 / (8 * np.pi * rh ** 3 * viscosity)], 1), 3,
+                      axis=1).ravel()
+        vk = d * F
+    if scale_brown > 0:
+        vk = vk + brownian_velocity(rh, dt, kBT, viscosity=viscosity, scale=scale_brown, seed=seed, step=step)
+    pairs = collision_pairs(pos, rc, colbuf, gid=gid)
+    gamma, fcol, vcol, info = collision_lcp(pairs, vk, dt, mob=mob, radius=rh, viscosity=viscosity, buffer=buffer,
+                                            tol=tol, max_iters=max_iters, gamma0=pairs["gamma"])
+    return vcol + vk, pairs, gamma, fcol, info
+

@@ -154,3 +154,35 @@ def test_collision_lcp_hydro_matches_oracle():
     assert np.linalg.eigvalsh(0.5 * (A + A.T)).min() > 0
     want = O.lcp_enumerate(A, b)
     assert np.abs(g - want).max() < 1e-6 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("hydro", [False, True])
+def test_resolve_collision_step(hydro):
+    """SphereSystem::resolveCollision end to end (known velocity + Brownian term,
+    pairs, LCP): the returned velocities keep every linearised gap
+    phi0 + dt D^T vel non-negative to the LCP tolerance, and pairs that push
+    (gamma > 0) close to contact; then one Euler step."""
+    say("resolve collision, hydro=%s" % hydro)
+    rng = np.random.default_rng(8)
+    n = 27
+    g = np.arange(3)
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) * 1.72 + rng.uniform(-0.02, 0.02, (n, 3))
+    rad = np.full(n, 0.8)
+    F = np.zeros((n, 6))
+    F[:, :3] = -(pos - pos.mean(0)) * 10.0
+    F[:, 3:] = rng.standard_normal((n, 3))
+    dt = 0.5
+    mob = None
+    if hydro:
+        mob = S.SphereSTKMobMat(pos, rad, np.tile([1.0, 0, 0, 0], (n, 1)), np.full(n, 6, dtype=np.int32))
+    vel, pairs, gamma, fcol, info = S.system.resolve_collision(pos, rad, F.ravel(), dt, 0.3, mob=mob, radius=rad,
+                                                               kBT=0.0041, scale_brown=1.0, seed=3, step=1,
+                                                               tol=1e-7)
+    assert info["converged"] == 1 and (gamma > 0).any()
+    v = vel.reshape(n, 6)[:, :3]
+    ii, jj, nrm = pairs["idx_i"], pairs["idx_j"], pairs["norm_i"]
+    phi = pairs["phi0"] + dt * np.einsum("kc,kc->k", nrm, v[ii] - v[jj])
+    assert phi.min() > -1e-6
+    assert np.abs(phi[gamma > 0]).max() < 1e-6
+    p2, q2 = S.system.step_euler(pos, np.tile([1.0, 0, 0, 0], (n, 1)), vel, dt)
+    assert np.isfinite(p2).all() and np.allclose(np.linalg.norm(q2, axis=1), 1)
