@@ -194,8 +194,8 @@ int sphstk_mobility_solve(sphstk_mobility *mob, const double *force, double *vel
 /* Several right-hand sides on the handle's geometry in one lock-step FP64
  * solve (the reference's apply asserts one column, MobMat.cpp:127): force and
  * vel hold ncol column blocks of 6N.  The traction far field is computed once
- * per point pair for up to 4 columns (d, r^2, r^-5 and d.n shared); more
- * columns go one at a time through the same launches.  Starts from x0 = 0 and
+ * per point pair for blocks of up to 8 columns (d, r^2, r^-5 and d.n shared).
+ * Starts from x0 = 0 and
  * leaves the handle's own warm start untouched; the first call for a given
  * ncol builds the copies (set-up).  No counterpart in the reference. */
 int sphstk_mobility_solve_multi(sphstk_mobility *mob, int ncol, const double *force, double *vel,

@@ -345,7 +345,7 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
             const std::vector<Runs> r0(runs_fine_.begin(), runs_fine_.begin() + s0);
             build_far_lists(lists_fine_, b0, so0, po0, po0, r0, 1, split, kWantItems);
             if (lists_fine_.split > 1) lists_fine_.part.alloc(size_t(lists_fine_.split) * 3 * npts_);
-            srcm_.alloc(std::max<int64_t>(4 * (ncopy_ + 1) * pt_off_[s0], 8));
+            srcm_.alloc(std::max<int64_t>(4 * (std::min(ncopy_, kMultiCols) + 1) * pt_off_[s0], 8));
         } else {
             build_far_lists(lists_fine_, nsusp_, susp_sph_off_, pt_off_, pt_off_, runs_fine_, tpl, split, kWantItems,
                             far_mfma_ ? pos_.data() : nullptr);
@@ -1058,9 +1058,9 @@ void Operator::far_levels(LevelSet &S, int kind, double coef, const double *in, 
 }
 
 // FP64 far field of ncopy identical copies of a set of suspensions (several
-// right-hand sides on one geometry), with the first copy's work lists: the
-// traction sum once per pair for all copies (k_far_trac_multi_f64, 2-4
-// copies); the single layer, or more copies, one copy at a time on the first
+// right-hand sides on one geometry), with the first copy's work lists.  The
+// traction sum is computed once per pair for blocks of up to 8 copies
+// (k_far_trac_multi_f64); the single layer one copy at a time on the first
 // copy's geometry.
 void Operator::far_copies(int kind, double coef, const double *in, double *out, const int *active, hipEvent_t ev_a) {
     FarLists &L = lists_fine_;
@@ -1069,10 +1069,13 @@ void Operator::far_copies(int kind, double coef, const double *in, double *out, 
     const bool split = L.split > 1;
     if (ev_a) SPHSTK_HIP(hipEventRecord(ev_a, stream_));
     if (L.pairs == 0.0) return;
-    if (kind == 0 && ncopy_ <= 4) {
-        launch_far_trac_multi(ncopy_, n0, d_coords_.p, d_norms_.p, d_w_.p, in, col, srcm_.p, L.ngroups, L.order.p,
-                              L.groups.p, L.iv.p, active, b0, split ? L.part.p : out, coef, split ? 0 : 1, 3 * npts_,
-                              stream_);
+    if (kind == 0) {
+        for (int c0 = 0; c0 < ncopy_; c0 += kMultiCols) {
+            const int nc = std::min(kMultiCols, ncopy_ - c0);
+            launch_far_trac_multi(nc, n0, d_coords_.p, d_norms_.p, d_w_.p, in + c0 * col, col, srcm_.p, L.ngroups,
+                                  L.order.p, L.groups.p, L.iv.p, active ? active + c0 * b0 : nullptr, b0,
+                                  (split ? L.part.p : out) + c0 * col, coef, split ? 0 : 1, 3 * npts_, stream_);
+        }
         if (split) launch_slot_reduce(out, L.part.p, L.split, 3 * npts_, chunks_.p, (int)chunks_.n, active, stream_);
         return;
     }

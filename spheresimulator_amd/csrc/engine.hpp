@@ -211,6 +211,7 @@ class Operator {
 
     int nsph_, nsusp_;
     int64_t npts_;
+    static constexpr int kMultiCols = 8; // right-hand sides per shared far-field launch
     int ncopy_ = 1;         // identical copies of one set of suspensions (several right-hand sides)
     DevBuf<double> srcm_;   // source records of the shared far field of the copies
     double c_[4];

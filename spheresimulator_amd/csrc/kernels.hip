@@ -1183,9 +1183,9 @@ void launch_centre_finish(int nsph, int nslot, const double *part, const double 
 }
 
 // ---------------------------------------------------------------------------
-// FP64 traction far field of NC densities on one geometry (NC right-hand
-// sides solved in lock step: copies of one set of suspensions).  The work
-// lists are those of the first copy; per source the record holds
+// FP64 traction far field of NC (1..8) densities on one geometry (NC
+// right-hand sides solved in lock step: copies of one set of suspensions).
+// The work lists are those of the first copy; per source the record holds
 // (x, y, z, 0, w f_0, 0, w f_1, 0, ...), 4 (NC + 1) doubles, so that d, r^2,
 // r^-5 and d.n are computed once per pair for all NC densities:
 // 15 VALU instructions + one v_rsq_f64 shared, 7 per density.
@@ -1248,20 +1248,22 @@ __global__ __launch_bounds__(64) void k_far_trac_multi_f64(const double *__restr
             az[c] = fmad(q, dz, az[c]);
         }
     };
+    // up to 4 columns: two sources per step, their wave-uniform (scalar) loads
+    // issued before the arithmetic of either; more columns: one (a record of
+    // 4 (NC + 1) doubles fills the scalar registers)
+    constexpr int U = NC <= 4 ? 2 : 1;
     for (int64_t k = G.iv_begin; k < G.iv_end; k++) {
         const int64_t b = iv[k].begin, e = iv[k].end;
         const bool incl = ((iv[k].mask[0] >> lane) & 1ull) != 0;
         int64_t s = b;
-        // two sources per step: their wave-uniform (scalar) loads are issued
-        // before the arithmetic of either
-        for (; s + 2 <= e; s += 2) {
-            double v[2][RS];
+        for (; s + U <= e; s += U) {
+            double v[U][RS];
 #pragma unroll
-            for (int u = 0; u < 2; u++)
+            for (int u = 0; u < U; u++)
 #pragma unroll
                 for (int j = 0; j < RS; j++) v[u][j] = src[RS * (s + u) + j];
-            pair(v[0], incl);
-            pair(v[1], incl);
+#pragma unroll
+            for (int u = 0; u < U; u++) pair(v[u], incl);
         }
         if (s < e) {
             double v[RS];
@@ -1288,32 +1290,40 @@ __global__ __launch_bounds__(64) void k_far_trac_multi_f64(const double *__restr
     }
 }
 
+template <int NC>
+static void far_trac_multi(int64_t n0, const double *coords, const double *norms, const double *w, const double *in,
+                           int64_t col_stride, double *src, int ngroups, const int *go, const FarGroup *gr,
+                           const FarInterval *iv, const int *active, int nsusp0, double *out, double coef, int acc,
+                           int64_t slot_stride, hipStream_t s) {
+    if (n0 > 0)
+        k_pack_src_multi<NC><<<dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, s>>>(n0, coords, w, in, col_stride,
+                                                                                     src);
+    if (ngroups > 0)
+        k_far_trac_multi_f64<NC><<<dim3(ngroups), dim3(64), 0, s>>>(src, coords, norms, go, gr, iv, active, nsusp0, out,
+                                                                   coef, acc, slot_stride, col_stride);
+}
+
 void launch_far_trac_multi(int ncol, int64_t n0, const double *coords, const double *norms, const double *w,
                            const double *in, int64_t col_stride, double *src, int ngroups, const int *go,
                            const FarGroup *gr, const FarInterval *iv, const int *active, int nsusp0, double *out,
                            double coef, int acc, int64_t slot_stride, hipStream_t s) {
-    const dim3 pb((unsigned)((n0 + 255) / 256)), pt(256);
     switch (ncol) {
-    case 2:
-        if (n0 > 0) k_pack_src_multi<2><<<pb, pt, 0, s>>>(n0, coords, w, in, col_stride, src);
-        if (ngroups > 0)
-            k_far_trac_multi_f64<2><<<dim3(ngroups), dim3(64), 0, s>>>(src, coords, norms, go, gr, iv, active, nsusp0,
-                                                                      out, coef, acc, slot_stride, col_stride);
+#define SPHSTK_MULTI_CASE(NC)                                                                                      \
+    case NC:                                                                                                       \
+        far_trac_multi<NC>(n0, coords, norms, w, in, col_stride, src, ngroups, go, gr, iv, active, nsusp0, out, coef, \
+                           acc, slot_stride, s);                                                                   \
         break;
-    case 3:
-        if (n0 > 0) k_pack_src_multi<3><<<pb, pt, 0, s>>>(n0, coords, w, in, col_stride, src);
-        if (ngroups > 0)
-            k_far_trac_multi_f64<3><<<dim3(ngroups), dim3(64), 0, s>>>(src, coords, norms, go, gr, iv, active, nsusp0,
-                                                                      out, coef, acc, slot_stride, col_stride);
-        break;
-    case 4:
-        if (n0 > 0) k_pack_src_multi<4><<<pb, pt, 0, s>>>(n0, coords, w, in, col_stride, src);
-        if (ngroups > 0)
-            k_far_trac_multi_f64<4><<<dim3(ngroups), dim3(64), 0, s>>>(src, coords, norms, go, gr, iv, active, nsusp0,
-                                                                      out, coef, acc, slot_stride, col_stride);
-        break;
+        SPHSTK_MULTI_CASE(1)
+        SPHSTK_MULTI_CASE(2)
+        SPHSTK_MULTI_CASE(3)
+        SPHSTK_MULTI_CASE(4)
+        SPHSTK_MULTI_CASE(5)
+        SPHSTK_MULTI_CASE(6)
+        SPHSTK_MULTI_CASE(7)
+        SPHSTK_MULTI_CASE(8)
+#undef SPHSTK_MULTI_CASE
     default:
-        throw Error(-1, "shared far field supports 2 to 4 right-hand sides");
+        throw Error(-1, "shared far field supports 1 to 8 right-hand sides per launch");
     }
 }
 

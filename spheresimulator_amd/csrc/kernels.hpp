@@ -128,9 +128,10 @@ void launch_slot_reduce(double *out, const double *part, int nslot, int64_t stri
 void launch_axpby(int64_t n, double a, const double *x, double b, double *y, hipStream_t s);
 void launch_fill(int64_t n, double v, double *y, hipStream_t s);
 
-// FP64 traction far field of ncol (2..4) densities on one geometry: `in` holds
+// FP64 traction far field of ncol (1..8) densities on one geometry: `in` holds
 // ncol DOF vectors col_stride apart, of which the first n0 points carry the
-// geometry and work lists; src is scratch of 4 (ncol + 1) doubles per point.
+// geometry and work lists; src is scratch of 4 (ncol + 1) doubles per point;
+// out (or its split slots, slot_stride apart) has the same column layout.
 void launch_far_trac_multi(int ncol, int64_t n0, const double *coords, const double *norms, const double *w,
                            const double *in, int64_t col_stride, double *src, int ngroups, const int *go,
                            const FarGroup *gr, const FarInterval *iv, const int *active, int nsusp0, double *out,

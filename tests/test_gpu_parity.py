@@ -369,11 +369,11 @@ def test_pole_values_and_vtu_output(tmp_path):
         assert np.allclose(np.linalg.norm(pts - pos[s], axis=1), rad[s], rtol=0, atol=1e-13)
 
 
-@pytest.mark.parametrize("ncol", [2, 3, 5])
+@pytest.mark.parametrize("ncol", [2, 3, 5, 9])
 def test_multi_rhs_solve_matches_single_solves(ncol):
     """Several right-hand sides on one geometry (two suspensions) in one lock-step
     FP64 solve: each column equals the oracle's solve of that column (the traction
-    far field shared by 2-4 columns; 5 columns go one at a time), a zero column
+    far field shared by blocks of up to 8 columns; 9 = 8 + 1), a zero column
     takes the zero-RHS shortcut, and the handle's own warm start is untouched."""
     say("multi-RHS solve ncol=%d" % ncol)
     geos = [small_geometry(seed=61, n=8, orders=(6, 8)), small_geometry(seed=62, n=6, orders=(7,))]
@@ -411,15 +411,15 @@ def test_multi_rhs_solve_matches_single_solves(ncol):
 
 
 def test_multi_rhs_solve_large_matches_sequential():
-    """The shared traction far field at a size with split source slices: four
-    columns in one solve against four single FP64 solves on the GPU."""
-    say("multi-RHS solve 256 spheres p=8, 4 columns")
+    """The shared traction far field at a size with split source slices: eight
+    columns in one solve against eight single FP64 solves on the GPU."""
+    say("multi-RHS solve 256 spheres p=8, 8 columns")
     c = S.configs.make_suspension(256, 8, seed=9, vf=0.15, random_orientation=True)
-    F = np.random.default_rng(80).standard_normal((4, 6 * len(c["radius"])))
+    F = np.random.default_rng(80).standard_normal((8, 6 * len(c["radius"])))
     gpu = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"])
     V, info = gpu.solveMob_multi(F)
     assert info["converged"] == 1
-    for k in range(4):
+    for k in range(8):
         gpu.set_unknown(np.zeros(gpu.op.n_dof))
         v, _ = gpu.solveMob(F[k])
         assert rel(V[k], v) < 1e-10
