@@ -208,6 +208,13 @@ int sphstk_mobility_apply(sphstk_mobility *mob, const double *X, double *Y, int 
 int sphstk_mobility_get_density(const sphstk_mobility *mob, double *rho);
 int sphstk_mobility_get_unknown(const sphstk_mobility *mob, double *x);
 int sphstk_mobility_set_unknown(sphstk_mobility *mob, const double *x);
+/* The constructor's warm start from the density stored on the spheres' layers
+ * (MobMat.cpp:67-93): x = projectNullSpace(rho).  The reference subtracts its
+ * member rho there, which is still zero, so rho0 stays in the guess; pass the
+ * density a previous handle returned by get_density (the values
+ * writeBackDensitySolution stores, :288-307) to reproduce its next step.
+ * set_unknown with the previous x is the closer guess. */
+int sphstk_mobility_set_density(sphstk_mobility *mob, const double *rho);
 
 /* ---- The steps beside the solve (host arrays; the work runs on the GPU) ---- */
 

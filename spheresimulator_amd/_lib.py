@@ -26,6 +26,7 @@ EXPORTED = [
     "sphstk_mobility_create", "sphstk_mobility_create_batched", "sphstk_mobility_destroy",
     "sphstk_mobility_operator", "sphstk_mobility_set_params", "sphstk_mobility_solve", "sphstk_mobility_solve_multi", "sphstk_mobility_apply",
     "sphstk_mobility_get_density", "sphstk_mobility_get_unknown", "sphstk_mobility_set_unknown",
+    "sphstk_mobility_set_density",
     "sphstk_brownian_velocity", "sphstk_collision_pairs", "sphstk_collision_lcp",
 ]
 
@@ -108,6 +109,7 @@ def load():
     L.sphstk_mobility_get_density.argtypes = [_vp, _dp]
     L.sphstk_mobility_get_unknown.argtypes = [_vp, _dp]
     L.sphstk_mobility_set_unknown.argtypes = [_vp, _dp]
+    L.sphstk_mobility_set_density.argtypes = [_vp, _dp]
     L.sphstk_brownian_velocity.argtypes = [ctypes.c_int, _dp] + [ctypes.c_double] * 4 + [ctypes.c_uint64] * 2 + [_dp]
     L.sphstk_collision_lcp.argtypes = [_vp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int64, _ip, _ip, _dp, _dp,
                                        _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp,

@@ -261,3 +261,9 @@ class SphereSTKMobMat:
     def set_unknown(self, x):
         x = _f64(x, self.op.n_dof, "x")
         check(load().sphstk_mobility_set_unknown(self._h, _d(x)))
+
+    def set_density(self, rho):
+        """Warm start from a stored density, x = projectNullSpace(rho), as the
+        reference's constructor does with the layer's gridValue (MobMat.cpp:67-93)."""
+        rho = _f64(rho, self.op.n_dof, "rho")
+        check(load().sphstk_mobility_set_density(self._h, _d(rho)))

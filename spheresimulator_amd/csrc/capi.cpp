@@ -450,6 +450,13 @@ int sphstk_mobility_set_unknown(sphstk_mobility *mob, const double *x) {
     });
 }
 
+int sphstk_mobility_set_density(sphstk_mobility *mob, const double *rho) {
+    return guard([&] {
+        SPHSTK_REQUIRE(mob && rho, "invalid arguments");
+        mob->mob->set_density(rho);
+    });
+}
+
 int sphstk_brownian_velocity(int n, const double *radius, double viscosity, double kBT, double dt, double scale,
                              uint64_t seed, uint64_t step, double *vel) {
     return guard([&] { brownian_velocity(n, radius, viscosity, kBT, dt, scale, seed, step, vel); });

@@ -1845,4 +1845,11 @@ void Mobility::set_unknown(const double *x) {
     SPHSTK_HIP(hipMemcpy(x_.p, x, sizeof(double) * A_->ndof(), hipMemcpyHostToDevice));
 }
 
+// The constructor's warm start from the density stored on the spheres
+// (BIE/SphereSTKMobMat.cpp:67-93): x = P(density), P the null-space projection.
+void Mobility::set_density(const double *rho) {
+    set_unknown(rho);
+    if (!x_zero_) A_->project(x_.p);
+}
+
 } // namespace sphstk

@@ -300,6 +300,8 @@ class Mobility {
     void get_density(double *rho) const;
     void get_unknown(double *x) const;
     void set_unknown(const double *x);
+    // warm start from a stored density: x = P(rho)
+    void set_density(const double *rho);
 
   private:
     // GMRES on the segments flagged in `act` (host, one per suspension); x and b device

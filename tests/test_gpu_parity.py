@@ -201,6 +201,29 @@ def test_warm_start_and_density():
     assert np.isfinite(rho).all() and rho.size == gpu.op.n_dof
 
 
+def test_set_density_warm_start():
+    """The constructor's warm start (MobMat.cpp:67-93): a new handle seeded with the
+    previous handle's density starts from x = projectNullSpace(rho) (rho0 is not
+    subtracted: the reference's member rho is still zero there) and converges to
+    the same velocities; a zero density is a cold start, bit for bit."""
+    say("set_density warm start")
+    pos, rad, quat, order = small_geometry(seed=33, n=6, orders=(6, 8))
+    F = np.random.default_rng(9).standard_normal(36)
+    a = S.SphereSTKMobMat(pos, rad, quat, order)
+    v1, i1 = a.solveMob(F)
+    rho = a.getDensitySolution()
+    b = S.SphereSTKMobMat(pos, rad, quat, order)
+    b.set_density(rho)
+    want = b.op.projectNullSpace(rho.copy())
+    assert np.abs(b.get_unknown() - want).max() <= 1e-14 * np.abs(want).max()
+    v2, i2 = b.solveMob(F)
+    assert i2["converged"] == 1 and rel(v2, v1) < 1e-9
+    c = S.SphereSTKMobMat(pos, rad, quat, order)
+    c.set_density(np.zeros_like(rho))
+    v3, i3 = c.solveMob(F)
+    assert i3["iters"] == i1["iters"] and np.array_equal(v3, v1)
+
+
 @pytest.mark.parametrize("p,direct,tol", [(8, True, 2e-6), (8, False, 2e-6), (12, False, 1e-6)])
 def test_inner_fp32_operator_accuracy(p, direct, tol):
     """The FP32 inner operator approximates the FP64 operator on a white band-
