@@ -142,6 +142,13 @@ int sphstk_operator_apply_lop(sphstk_operator *op, const double *in, double *out
  * which feeds the grid-with-poles output of Shexp::writeVTU, :96-204). */
 int sphstk_operator_pole_values(sphstk_operator *op, const double *in, double *poles);
 
+/* out = M^-1 in, M the block-diagonal part of the mobility operator
+ * 1/2 I + K' + L (each sphere's self operator, the far and near terms
+ * dropped), inverted exactly per sphere and SH degree.  The operator must have
+ * the coefficients (0.5, any, 1, 1).  The mixed form's inner solve uses it as a
+ * right preconditioner.  No counterpart in the reference (Belos runs
+ * unpreconditioned there). */
+int sphstk_operator_apply_self_inverse(sphstk_operator *op, const double *in, double *out);
 /* projectNullSpace: in-place projection onto vector SH of degree <= p
  * (cpp:239-260; hpp:200-202). */
 int sphstk_operator_project_nullspace(sphstk_operator *op, double *inout);

@@ -21,7 +21,7 @@ EXPORTED = [
     "sphstk_operator_sizes", "sphstk_operator_apply", "sphstk_operator_apply_device", "sphstk_operator_apply_inner",
     "sphstk_operator_apply_outer",
     "sphstk_operator_apply_p2p", "sphstk_operator_apply_p2p_device", "sphstk_operator_apply_lop",
-    "sphstk_operator_project_nullspace", "sphstk_operator_pole_values", "sphstk_operator_get_grid", "sphstk_operator_get_near_pairs",
+    "sphstk_operator_project_nullspace", "sphstk_operator_apply_self_inverse", "sphstk_operator_pole_values", "sphstk_operator_get_grid", "sphstk_operator_get_near_pairs",
     "sphstk_operator_far_shape", "sphstk_operator_kernel_stats", "sphstk_operator_reset_stats",
     "sphstk_mobility_create", "sphstk_mobility_create_batched", "sphstk_mobility_destroy",
     "sphstk_mobility_operator", "sphstk_mobility_set_params", "sphstk_mobility_solve", "sphstk_mobility_solve_multi", "sphstk_mobility_apply",
@@ -89,6 +89,7 @@ def load():
     L.sphstk_operator_apply_p2p_device.argtypes = [_vp, _vp, _vp] + [ctypes.c_double] * 3
     L.sphstk_operator_apply_lop.argtypes = [_vp, _dp, _dp, ctypes.c_double]
     L.sphstk_operator_project_nullspace.argtypes = [_vp, _dp]
+    L.sphstk_operator_apply_self_inverse.argtypes = [_vp, _dp, _dp]
     L.sphstk_operator_get_grid.argtypes = [_vp, _dp, _dp, _dp, _i64p, _i64p]
     L.sphstk_operator_get_near_pairs.argtypes = [_vp, _i32p, _i32p]
     L.sphstk_operator_kernel_stats.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _i64p, _dp]

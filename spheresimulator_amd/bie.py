@@ -124,6 +124,14 @@ class SphereSTKSHOperator:
         check(load().sphstk_operator_apply_lop(self._h, _d(inp), _d(out), cLOP))
         return out
 
+    def apply_self_inverse(self, x):
+        """M^-1 x with M each sphere's self operator 1/2 I + K'_self P + L (exact inverse per
+        sphere and SH degree; the mixed form's inner preconditioner)."""
+        x = _f64(x, self.n_dof, "x")
+        y = np.zeros(self.n_dof)
+        check(load().sphstk_operator_apply_self_inverse(self._h, _d(x), _d(y)))
+        return y
+
     def projectNullSpace(self, inout):
         if inout.size != self.n_dof or inout.dtype != np.float64:
             raise ValueError("inout must be a float64 array of length n_dof")

@@ -275,6 +275,22 @@ int sphstk_operator_pole_values(sphstk_operator *op, const double *in, double *p
     });
 }
 
+int sphstk_operator_apply_self_inverse(sphstk_operator *op, const double *in, double *out) {
+    return guard([&] {
+        SPHSTK_REQUIRE(op && op->op, "invalid operator handle");
+        SPHSTK_REQUIRE(in && out, "in and out must not be NULL");
+        Operator &A = *op->op;
+        const int64_t n = A.ndof();
+        DevBuf<double> a, b;
+        a.alloc(std::max<int64_t>(n, 1));
+        b.alloc(std::max<int64_t>(n, 1));
+        SPHSTK_HIP(hipMemcpyAsync(a.p, in, sizeof(double) * n, hipMemcpyHostToDevice, A.stream()));
+        A.precondition(a.p, b.p, nullptr);
+        SPHSTK_HIP(hipMemcpyAsync(out, b.p, sizeof(double) * n, hipMemcpyDeviceToHost, A.stream()));
+        SPHSTK_HIP(hipStreamSynchronize(A.stream()));
+    });
+}
+
 int sphstk_operator_project_nullspace(sphstk_operator *op, double *inout) {
     return guard([&] {
         SPHSTK_REQUIRE(op && op->op, "invalid operator handle");

@@ -1430,6 +1430,55 @@ void Operator::project(double *inout) {
     SPHSTK_HIP(hipGetLastError());
 }
 
+// Blocks (B_n^-1 - 2 I) per SH degree n of the single-sphere operator
+// B = 1/2 I + K'_PV P + L restricted to degree-n coefficients (F_r, F_s, F_t):
+// B_n = I + K_ext(n) + L(n) with the exterior traction spectra of a sphere
+//   K_ext: (F_r, F_s) -> -(n (4n^2+6n-1) F_r + 3n(n+1) F_s, 3 F_r + (4n^3+6n^2-n+3) F_s) / D_n,
+//          F_t -> -(n+2)/(2n+1) F_t,  D_n = (2n-1)(2n+1)(2n+3),
+// and the rigid-body projector L on degree 1 (translations F_r = F_s, rotations
+// F_t): (F_r, F_s) -> (F_r + 2 F_s)/3 (1, 1), F_t -> F_t.  Degree 0 (the normal
+// field) and the part of a density beyond degree p see 1/2 I, hence the "- 2 I":
+// M^-1 x = 2 x + synthesis((B_n^-1 - 2 I) analysis(x)).  Scale-free in R.
+static std::vector<double> self_inverse_blocks() {
+    std::vector<double> t(5 * (kMaxOrder + 1));
+    t[0] = -1.0;
+    t[3] = -1.0;
+    t[4] = -1.0;
+    for (int n = 1; n <= kMaxOrder; n++) {
+        const double nn = n, D = (2 * nn - 1) * (2 * nn + 1) * (2 * nn + 3);
+        double a = 1.0 - nn * (4 * nn * nn + 6 * nn - 1) / D, b = -3.0 * nn * (nn + 1) / D, c = -3.0 / D,
+               e = 1.0 - (4 * nn * nn * nn + 6 * nn * nn - nn + 3) / D, bt = 1.0 - (nn + 2) / (2 * nn + 1);
+        if (n == 1) {
+            a += 1.0 / 3.0;
+            b += 2.0 / 3.0;
+            c += 1.0 / 3.0;
+            e += 2.0 / 3.0;
+            bt += 1.0;
+        }
+        const double det = a * e - b * c;
+        t[5 * n + 0] = e / det - 2.0;
+        t[5 * n + 1] = -b / det;
+        t[5 * n + 2] = -c / det;
+        t[5 * n + 3] = a / det - 2.0;
+        t[5 * n + 4] = 1.0 / bt - 2.0;
+    }
+    return t;
+}
+
+// out = M^-1 in, M the block-diagonal (one block per sphere) part of the
+// operator: the exact inverse of each sphere's self operator (isolated-sphere
+// preconditioner).  Suspensions not active keep out = 2 in.
+void Operator::precondition(const double *in, double *out, const int *active) {
+    if (nsph_ == 0) return;
+    if (!self_inv_.p) self_inv_.upload(self_inverse_blocks());
+    launch_analysis(geom_, in, coef_.p, active, stream_, max_order_);
+    launch_degree_blocks(geom_, coef_.p, self_inv_.p, active, stream_);
+    launch_expand(geom_, 1, coef_.p, rec_.p, active, stream_, max_order_);
+    launch_fill(3 * npts_, 0.0, out, stream_);
+    launch_synthesis(geom_, rec_.p, self_items_.p, (int)self_items_.n, out, stream_, active);
+    launch_axpby(3 * npts_, 2.0, in, 1.0, out, stream_);
+}
+
 void Operator::grid(double *weights, double *coords, double *norms, int64_t *index, int64_t *length) const {
     if (weights) std::copy(w_.begin(), w_.end(), weights);
     if (coords) std::copy(coords_.begin(), coords_.end(), coords);
@@ -1479,6 +1528,7 @@ Mobility::Mobility(int nsusp, const int *counts, const double *pos, const double
     : mu_(viscosity), prm_(prm), prec_(precision) {
     if (const char *v = std::getenv("SPHSTK_OUTER_ML")) outer_prec_ = std::atoi(v) != 0 ? 2 : 0;
     inner_restart_ = std::max(prm.restart, 60);
+    if (const char *v = std::getenv("SPHSTK_INNER_PRECOND")) inner_precond_ = std::atoi(v) != 0;
     if (const char *v = std::getenv("SPHSTK_INNER_RESTART")) inner_restart_ = std::atoi(v);
     SPHSTK_REQUIRE(inner_restart_ >= 1 && inner_restart_ <= 1000, "SPHSTK_INNER_RESTART must lie in [1, 1000]");
     SPHSTK_REQUIRE(std::isfinite(viscosity) && viscosity > 0, "viscosity must be > 0");
@@ -1533,6 +1583,13 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
     const int nch = A.nchunks();
     const int *sco = A.seg_chunk_off();
     double *part = A.scratch_partial();
+    // inner solve: right preconditioning by the isolated-sphere inverse,
+    // A M^-1 u = r, d = M^-1 u (the residual stays the unpreconditioned one)
+    const bool rprec = prec == 1 && inner_precond_;
+    if (rprec) {
+        z_.alloc(std::max<int64_t>(n, 1));
+        zt_.alloc(std::max<int64_t>(n, 1));
+    }
 
     std::vector<double> bn, beta;
     seg_norms(b, bn);
@@ -1585,7 +1642,10 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
         for (int j = 0; j < m; j++) {
             double *W = V_.p + size_t(j + 1) * n;
             const double *Vj = V_.p + size_t(j) * n;
-            if (prec == 1)
+            if (rprec) {
+                A.precondition(Vj, z_.p, act_.p);
+                A.apply_inner_into(z_.p, W, act_.p);
+            } else if (prec == 1)
                 A.apply_inner_into(Vj, W, act_.p);
             else
                 A.apply_into(Vj, W, prec, act_.p);
@@ -1682,7 +1742,15 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
         }
         upload_act(cyc);
         SPHSTK_HIP(hipMemcpyAsync(coefv_.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, st));
-        launch_seg_multi_axpy(x, V_.p, n, coefv_.p, kmax, B, ch, nch, act_.p, st);
+        if (rprec) {
+            // x += M^-1 (V y); zero outside the suspensions of this cycle
+            launch_fill(n, 0.0, zt_.p, st);
+            launch_seg_multi_axpy(zt_.p, V_.p, n, coefv_.p, kmax, B, ch, nch, act_.p, st);
+            A.precondition(zt_.p, z_.p, act_.p);
+            launch_axpby(n, 1.0, z_.p, 1.0, x, st);
+        } else {
+            launch_seg_multi_axpy(x, V_.p, n, coefv_.p, kmax, B, ch, nch, act_.p, st);
+        }
         bool need_res = prm_.explicit_check != 0;
         for (int s = 0; s < B; s++)
             if (cyc[s] && !impok[s]) need_res = true;

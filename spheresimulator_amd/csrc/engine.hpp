@@ -160,6 +160,8 @@ class Operator {
     // from a HIP graph captured on the first call for this `active` array
     void apply_inner_into(const double *in, double *out, const int *active);
     void project(double *inout);
+    // out = M^-1 in with M the per-sphere self operator (exact inverse per sphere)
+    void precondition(const double *in, double *out, const int *active);
     // SH expansion of in (device) at the two poles of every sphere, lab frame, 6 per sphere (host out)
     void pole_values(const double *in, double *out);
     // rigid-body velocities (6 per sphere) of the single layer of rho, distant pairs by Faxen's laws
@@ -214,6 +216,7 @@ class Operator {
     static constexpr int kMultiCols = 8; // right-hand sides per shared far-field launch
     int ncopy_ = 1;         // identical copies of one set of suspensions (several right-hand sides)
     DevBuf<double> srcm_;   // source records of the shared far field of the copies
+    DevBuf<double> self_inv_; // per-degree blocks of the isolated-sphere inverse
     double c_[4];
     int flags_;
     int max_order_ = 0;
@@ -315,6 +318,8 @@ class Mobility {
     int prec_;
     int outer_prec_ = 2; // far field of the mixed form's FP64 applies: 2 multi-level, 0 direct
     int inner_restart_ = 30; // GMRES restart of the inner FP32-operator solve
+    bool inner_precond_ = false; // right-precondition the inner solve by the isolated-sphere inverse
+    DevBuf<double> z_, zt_;      // preconditioned basis vector / update
     bool x_zero_ = true; // the stored unknown is known to be zero (cold start)
     bool verbose_ = false; // SPHSTK_VERBOSE=1: residual history on stderr (Belos Verbosity 63)
     DevBuf<double> rho_, b_, x_, r_, d_, V_, h_, scal_, coefv_, force_, vel_, gmres_inner_rhs_, mdot_;

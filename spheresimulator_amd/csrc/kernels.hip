@@ -697,6 +697,32 @@ __global__ __launch_bounds__(256) void k_expand(DevGeom g, int mode, const doubl
     }
 }
 
+// Per-degree blocks on the SH coefficients of each sphere, in place:
+// (F_r, F_s) <- M_n (F_r, F_s) (real 2x2, tab[5n..5n+3] row-major),
+// F_t <- tab[5n+4] F_t.  Used for the inverse of the single-sphere operator.
+__global__ __launch_bounds__(256) void k_degree_blocks(DevGeom g, double *__restrict__ coef,
+                                                       const double *__restrict__ tab,
+                                                       const int *__restrict__ active) {
+    const int S = blockIdx.x;
+    if (active && !active[g.sph_susp[S]]) return;
+    const int p = g.order[S];
+    const int Q = (p + 1) * (p + 2) / 2;
+    double2 *F = reinterpret_cast<double2 *>(coef + g.coef_off[S]);
+    for (int k = threadIdx.x; k < Q; k += blockDim.x) {
+        int n = 0;
+        while ((n + 1) * (n + 2) / 2 <= k) n++;
+        const double *t = tab + 5 * n;
+        const double2 r = F[k], s = F[Q + k], u = F[2 * Q + k];
+        F[k] = make_double2(t[0] * r.x + t[1] * s.x, t[0] * r.y + t[1] * s.y);
+        F[Q + k] = make_double2(t[2] * r.x + t[3] * s.x, t[2] * r.y + t[3] * s.y);
+        F[2 * Q + k] = make_double2(t[4] * u.x, t[4] * u.y);
+    }
+}
+
+void launch_degree_blocks(const DevGeom &g, double *coef, const double *tab, const int *active, hipStream_t s) {
+    if (g.nsph) k_degree_blocks<<<dim3(g.nsph), dim3(256), 0, s>>>(g, coef, tab, active);
+}
+
 void launch_expand(const DevGeom &g, int mode, const double *coef, double *rec, const int *active, hipStream_t s,
                    int max_order) {
     if (!g.nsph) return;
@@ -874,9 +900,9 @@ void launch_sheval(const DevGeom &g, int mode, const double *rec, const EvalItem
 }
 
 void launch_synthesis(const DevGeom &g, const double *rec, const EvalItem *self_items, int nitems, double *out,
-                      hipStream_t s) {
+                      hipStream_t s, const int *active) {
     if (!g.nsph || nitems <= 0) return;
-    k_sheval<2><<<dim3(nitems), dim3(64), 0, s>>>(g, rec, self_items, nullptr, out, 0.0, 0.0, nullptr);
+    k_sheval<2><<<dim3(nitems), dim3(64), 0, s>>>(g, rec, self_items, nullptr, out, 0.0, 0.0, active);
 }
 
 __global__ void k_identity(DevGeom g, const double *__restrict__ in, double *__restrict__ out, double cId,

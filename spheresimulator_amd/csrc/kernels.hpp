@@ -77,6 +77,8 @@ void launch_resample(const DevGeom &g, const ResampleTabs &tab, const int64_t *i
 
 void launch_analysis(const DevGeom &g, const double *in, double *coef, const int *active, hipStream_t s,
                      int max_order);
+// per sphere and degree n: (F_r, F_s) <- 2x2 block tab[5n..5n+3], F_t <- tab[5n+4] F_t (in place)
+void launch_degree_blocks(const DevGeom &g, double *coef, const double *tab, const int *active, hipStream_t s);
 // mode 0: single-layer record (kRecSL arrays), 1: traction record (kRecTrac arrays)
 void launch_expand(const DevGeom &g, int mode, const double *coef, double *rec, const int *active, hipStream_t s,
                    int max_order);
@@ -98,7 +100,7 @@ void launch_identity(const DevGeom &g, const double *in, double *out, double cId
 void launch_lop(const DevGeom &g, const double *in, double *out, double cLOP, const int *active, hipStream_t s);
 // out = P in (grid values of the band-limited projection) from a traction record (self items only)
 void launch_synthesis(const DevGeom &g, const double *rec, const EvalItem *self_items, int nitems, double *out,
-                      hipStream_t s);
+                      hipStream_t s, const int *active = nullptr);
 
 // segmented BLAS-1 over chunks (one segment per suspension)
 void launch_seg_dot(const double *a, const double *b, const Chunk *chunks, int nchunks, double *partial,
