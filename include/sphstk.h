@@ -198,11 +198,12 @@ int sphstk_mobility_set_params(sphstk_mobility *mob, const sphstk_solver_params 
  * unknown x (xLast, MobMat.cpp:222, :245); info may be NULL.  In batched form
  * every suspension is solved; info aggregates (iters = max over suspensions). */
 int sphstk_mobility_solve(sphstk_mobility *mob, const double *force, double *vel, sphstk_solve_info *info);
-/* Several right-hand sides on the handle's geometry in one lock-step FP64
- * solve (the reference's apply asserts one column, MobMat.cpp:127): force and
- * vel hold ncol column blocks of 6N.  The traction far field is computed once
- * per point pair for blocks of up to 8 columns (d, r^2, r^-5 and d.n shared).
- * Starts from x0 = 0 and
+/* Several right-hand sides on the handle's geometry in one lock-step solve
+ * (the reference's apply asserts one column, MobMat.cpp:127): force and vel
+ * hold ncol column blocks of 6N.  FP64 form: the traction far field is
+ * computed once per point pair for blocks of up to 8 columns (d, r^2, r^-5 and
+ * d.n shared).  Mixed form: the columns are solved as a batch of independent
+ * copies (no shared pair sums).  Starts from x0 = 0 and
  * leaves the handle's own warm start untouched; the first call for a given
  * ncol builds the copies (set-up).  No counterpart in the reference. */
 int sphstk_mobility_solve_multi(sphstk_mobility *mob, int ncol, const double *force, double *vel,

@@ -26,7 +26,7 @@ struct sphstk_mobility {
     double viscosity = 1.0;
     int flags = 0;
     std::unique_ptr<Mobility> multi; // copies of the suspensions, one per right-hand side
-    int multi_ncol = 0;
+    int multi_ncol = 0, multi_prec = 0;
 };
 
 namespace {
@@ -418,9 +418,9 @@ int sphstk_mobility_solve_multi(sphstk_mobility *mob, int ncol, const double *fo
     return guard([&] {
         SPHSTK_REQUIRE(mob && force && vel, "invalid arguments");
         SPHSTK_REQUIRE(ncol >= 1 && ncol <= 64, "ncol must lie in [1, 64]");
-        SPHSTK_REQUIRE(mob->mob->precision() == 0, "several right-hand sides: FP64 form only");
+        const int prec = mob->mob->precision();
         const int B = (int)mob->counts.size();
-        if (!mob->multi || mob->multi_ncol != ncol) {
+        if (!mob->multi || mob->multi_ncol != ncol || mob->multi_prec != prec) {
             mob->multi.reset();
             std::vector<int> counts, order;
             std::vector<double> pos, radius, quat;
@@ -431,12 +431,15 @@ int sphstk_mobility_solve_multi(sphstk_mobility *mob, int ncol, const double *fo
                 radius.insert(radius.end(), mob->radius.begin(), mob->radius.end());
                 quat.insert(quat.end(), mob->quat.begin(), mob->quat.end());
             }
+            // FP64: copies that share the traction pair sums; mixed: the copies
+            // as independent suspensions of one batched solve
             mob->multi = std::make_unique<Mobility>(ncol * B, counts.data(), pos.data(), radius.data(), quat.data(),
-                                                    order.data(), mob->viscosity, mob->mob->params(), 0,
-                                                    mob->flags & ~4, ncol);
+                                                    order.data(), mob->viscosity, mob->mob->params(), prec,
+                                                    mob->flags & ~4, prec == 0 ? ncol : 1);
             mob->multi_ncol = ncol;
+            mob->multi_prec = prec;
         }
-        mob->multi->set_params(mob->mob->params(), 0);
+        mob->multi->set_params(mob->mob->params(), prec);
         std::vector<SegInfo> seg;
         int outer = 0, a64 = 0, a32 = 0;
         mob->multi->set_unknown_zero();

@@ -128,7 +128,7 @@ struct SegInfo {
 class Operator {
   public:
     // ncopy > 1: the nsusp suspensions are ncopy identical copies of the first
-    // nsusp / ncopy (several right-hand sides on one geometry, FP64 only): the
+    // nsusp / ncopy (several right-hand sides on one geometry, FP64 form): the
     // far field is computed once per pair for all copies.
     Operator(int nsusp, const int *counts, const double *pos, const double *radius, const double *quat,
              const int *order, double cId, double cSL, double cTrac, double cLOP, int flags, int ncopy = 1);

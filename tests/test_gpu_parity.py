@@ -433,6 +433,22 @@ def test_multi_rhs_solve_matches_single_solves(ncol):
     assert rel(v1, v0) < 1e-12
 
 
+def test_multi_rhs_mixed_form():
+    """The mixed form takes several right-hand sides as a batch of copies: each
+    column meets the mixed bar (1e-7) against its own FP64 solve."""
+    say("multi-RHS solve, mixed form, 3 columns")
+    pos, rad, quat, order = small_geometry(seed=71, n=8, orders=(6, 8))
+    F = np.random.default_rng(72).standard_normal((3, 48))
+    mix = S.SphereSTKMobMat(pos, rad, quat, order, precision=S.PREC_MIXED)
+    V, info = mix.solveMob_multi(F)
+    assert info["converged"] == 1 and info["applies_fp32"] > 0
+    ref = S.SphereSTKMobMat(pos, rad, quat, order)
+    for k in range(3):
+        ref.set_unknown(np.zeros(ref.op.n_dof))
+        v, _ = ref.solveMob(F[k])
+        assert rel(V[k], v) < 1e-7
+
+
 def test_multi_rhs_solve_large_matches_sequential():
     """The shared traction far field at a size with split source slices: eight
     columns in one solve against eight single FP64 solves on the GPU."""
