@@ -202,6 +202,9 @@ struct Stream {
 };
 
 unsigned grid_of(int n) { return (unsigned)((n + 255) / 256); }
+// the pair passes run one wave per block, so that even 16k spheres spread over
+// all 256 CUs (one thread per sphere, latency-bound cell scans)
+unsigned grid_of64(int n) { return (unsigned)((n + 63) / 64); }
 
 } // namespace
 
@@ -288,7 +291,7 @@ int64_t collision_pairs(int n, const double *pos, const double *rcol, const int 
     k_fill_i32<<<cg, 256, 0, st.s>>>(ncells, 0, end.p);
     SPHSTK_HIP(hipMemsetAsync(count.p + n, 0, sizeof(int64_t), st.s));
     k_cell_bounds<<<grid_of(n), 256, 0, st.s>>>(n, skey.p, begin.p, end.p);
-    k_pairs<false><<<grid_of(n), 256, 0, st.s>>>(n, dpos.p, drc.p, dgid.p, colbuf, g, sorted.p, begin.p, end.p,
+    k_pairs<false><<<grid_of64(n), 64, 0, st.s>>>(n, dpos.p, drc.p, dgid.p, colbuf, g, sorted.p, begin.p, end.p,
                                                  count.p, nullptr, nullptr, nullptr, nullptr, nullptr);
     SPHSTK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_scan, count.p, off.p, n + 1, st.s));
     SPHSTK_HIP(hipGetLastError());
@@ -299,7 +302,7 @@ int64_t collision_pairs(int n, const double *pos, const double *rcol, const int 
     if (!want || total > capacity || total == 0) return total;
     Dev<int> dii(total), djj(total);
     Dev<double> dphi(total), dnorm(3 * (size_t)total);
-    k_pairs<true><<<grid_of(n), 256, 0, st.s>>>(n, dpos.p, drc.p, dgid.p, colbuf, g, sorted.p, begin.p, end.p,
+    k_pairs<true><<<grid_of64(n), 64, 0, st.s>>>(n, dpos.p, drc.p, dgid.p, colbuf, g, sorted.p, begin.p, end.p,
                                                 nullptr, off.p, dii.p, djj.p, dphi.p, dnorm.p);
     SPHSTK_HIP(hipGetLastError());
     if (ii) SPHSTK_HIP(hipMemcpyAsync(ii, dii.p, sizeof(int) * total, hipMemcpyDeviceToHost, st.s));
