@@ -55,3 +55,16 @@ def test_invalid_inputs_rejected_before_device_use():
     # validation happens before any device work: wrong sizes are caught in Python
     with pytest.raises(ValueError):
         S.SphereSTKSHOperator(np.zeros((2, 3)), [1.0], None, [4])
+
+
+def test_header_and_library_from_c(tmp_path):
+    """include/sphstk.h is plain C: a C99 program includes it, links
+    libsphstk.so and runs the entry points that need no device."""
+    exe = tmp_path / "abi_from_c"
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "c", "abi_from_c.c"), "-o", str(exe), "-L", libdir, "-lsphstk",
+                    "-Wl,-rpath," + libdir, "-lm"], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip() == "ok"
