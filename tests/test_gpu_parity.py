@@ -501,3 +501,37 @@ def test_self_inverse_preconditioned_mixed_solve():
         else:
             os.environ["SPHSTK_INNER_PRECOND"] = old
     assert its["1"] < its["0"]
+
+
+@pytest.mark.parametrize("prec,tol", [(S.PREC_FP64, 1e-10), (S.PREC_MIXED, 1e-7)])
+def test_batched_with_empty_and_single_sphere_suspensions(prec, tol):
+    """Batches may hold empty suspensions and one-sphere suspensions (no far or near
+    pairs): every non-empty one matches its own oracle solve, the single sphere
+    the closed form V = F/(6 pi mu R), Omega = T/(8 pi mu R^3)."""
+    say("batched mobility with empty and single-sphere suspensions, prec=%d" % prec)
+    g1 = small_geometry(seed=81, n=4, orders=(6,))
+    g2 = (np.array([[5.0, 5.0, 5.0]]), np.array([0.7]), np.array([quat_of([1, 1, 0], 0.4)]),
+          np.array([8], dtype=np.int32))
+    g3 = small_geometry(seed=82, n=5, orders=(7, 6))
+    parts = [g1, None, g2, g3, None]
+    counts = [0 if g is None else len(g[1]) for g in parts]
+    geos = [g for g in parts if g is not None]
+    pos = np.concatenate([g[0] for g in geos])
+    rad = np.concatenate([g[1] for g in geos])
+    quat = np.concatenate([g[2] for g in geos])
+    order = np.concatenate([g[3] for g in geos])
+    F = np.random.default_rng(83).standard_normal(6 * len(rad))
+    gpu = S.SphereSTKMobMat(pos, rad, quat, order, counts=counts, precision=prec)
+    v, info = gpu.solveMob(F)
+    assert info["converged"] == 1
+    o = 0
+    for g in geos:
+        k = len(g[1])
+        if k == 1:
+            R = g[1][0]
+            want = np.concatenate([F[6 * o:6 * o + 3] / (6 * np.pi * R), F[6 * o + 3:6 * o + 6] / (8 * np.pi * R ** 3)])
+            assert rel(v[6 * o:6 * o + 6], want) < (1e-12 if prec == S.PREC_FP64 else tol)
+        else:
+            vr, _ = O.MobilityOperator(*g).solve(F[6 * o:6 * (o + k)], tol=1e-8)
+            assert rel(v[6 * o:6 * (o + k)], vr) < tol
+        o += k
