@@ -715,14 +715,25 @@ void orc_far_sums_range(const orc_op *op, const double *in, double *out, double 
  * (:340-348, :417-419, :452) and the near terms are the exterior SH evaluations
  * of the near sources' projected densities (:575-695). */
 void orc_apply_p2p(const orc_op *op, const double *in, double *out, double cId, double cSL, double cTrac) {
+    orc_apply_p2p_spheres(op, in, out, cId, cSL, cTrac, 0, op->nsph);
+}
+
+/* applyP2POP on the points of target spheres [T0, T1) only: the same terms as
+ * orc_apply_p2p (all spheres act as sources), written into those spheres'
+ * entries of out; the other entries are left untouched. */
+void orc_apply_p2p_spheres(const orc_op *op, const double *in, double *out, double cId, double cSL, double cTrac,
+                           int T0, int T1) {
     const int doSL = fabs(cSL) > 1e-9, doT = fabs(cTrac) > 1e-9;
+    if (T0 < 0) T0 = 0;
+    if (T1 > op->nsph) T1 = op->nsph;
+    if (T1 <= T0) return;
     if (doSL || doT) {
         double *coef = (double *)malloc(sizeof(double) * op->coef_off[op->nsph]);
         all_coefficients(op, in, coef);
-        orc_far_sums(op, in, out, cSL, cTrac);
+        orc_far_sums_range(op, in, out, cSL, cTrac, op->off[T0], op->off[T1]);
         /* self */
 #pragma omp parallel for schedule(dynamic, 4)
-        for (int i = 0; i < op->nsph; i++) {
+        for (int i = T0; i < T1; i++) {
             const int p = op->order[i];
             const long n = op->off[i + 1] - op->off[i];
             const long nc = 3L * (p + 1) * (p + 2);
@@ -750,7 +761,7 @@ void orc_apply_p2p(const orc_op *op, const double *in, double *out, double cId, 
         }
         /* near */
 #pragma omp parallel for schedule(dynamic, 1)
-        for (int T = 0; T < op->nsph; T++) {
+        for (int T = T0; T < T1; T++) {
             const long n = op->off[T + 1] - op->off[T];
             double *xi = (double *)malloc(sizeof(double) * 3 * n), *nu = (double *)malloc(sizeof(double) * 3 * n);
             double *u = (double *)malloc(sizeof(double) * 3 * n), *t = (double *)malloc(sizeof(double) * 3 * n);
@@ -787,7 +798,7 @@ void orc_apply_p2p(const orc_op *op, const double *in, double *out, double cId, 
         free(coef);
     }
     if (fabs(cId) > 1e-9) {
-        for (long k = 0; k < 3 * op->npts; k++) out[k] += cId * in[k];
+        for (long k = 3 * op->off[T0]; k < 3 * op->off[T1]; k++) out[k] += cId * in[k];
     }
 }
 

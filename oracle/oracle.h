@@ -75,6 +75,9 @@ void orc_get_near_pairs(const orc_op *op, int *trg, int *src);
 
 /* out += cId*in + cSL*S[in] + cTrac*K'[in]   (SphereSTKSHOperator.cpp:262-485) */
 void orc_apply_p2p(const orc_op *op, const double *in, double *out, double cId, double cSL, double cTrac);
+/* The same on the points of target spheres [T0, T1) only (all spheres are sources). */
+void orc_apply_p2p_spheres(const orc_op *op, const double *in, double *out, double cId, double cSL, double cTrac,
+                           int T0, int T1);
 /* out += cLOP * L[in]                        (SphereSTKSHOperator.cpp:487-521) */
 void orc_apply_lop(const orc_op *op, const double *in, double *out, double cLOP);
 /* Y = beta*Y + alpha*A*X with the stored coefficients (:153-236) */

@@ -59,6 +59,7 @@ def lib():
         L.orc_num_near_pairs.restype = ctypes.c_long
         L.orc_get_near_pairs.argtypes = [ctypes.c_void_p, _ip, _ip]
         L.orc_apply_p2p.argtypes = [ctypes.c_void_p, _dp, _dp] + [ctypes.c_double] * 3
+        L.orc_apply_p2p_spheres.argtypes = [ctypes.c_void_p, _dp, _dp] + [ctypes.c_double] * 3 + [ctypes.c_int] * 2
         L.orc_apply_lop.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double]
         L.orc_apply.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.c_double]
         L.orc_project.argtypes = [ctypes.c_void_p, _dp]
@@ -189,6 +190,14 @@ class Operator:
         x = _c(x).ravel()
         out = np.zeros(self.dof) if out is None else out
         lib().orc_apply_p2p(self._h, _d(x), _d(out), cId, cSL, cTrac)
+        return out
+
+    def apply_p2p_spheres(self, x, T0, T1, out=None, cId=0.0, cSL=0.0, cTrac=0.0):
+        """apply_p2p evaluated on the points of target spheres [T0, T1) only;
+        the other entries of out are left as they were."""
+        x = _c(x).ravel()
+        out = np.zeros(self.dof) if out is None else out
+        lib().orc_apply_p2p_spheres(self._h, _d(x), _d(out), cId, cSL, cTrac, int(T0), int(T1))
         return out
 
     def apply_lop(self, x, out=None, cLOP=1.0):

@@ -1,0 +1,142 @@
+"""Parity at the full size of the 1024-sphere, p=12 workload (vf 0.05, 346,112
+grid points, 1.04 M unknowns): the oracle cannot run a full solve there in test
+time, so its sphere-range apply checks the GPU's results on sampled target
+spheres (all 1024 spheres act as sources):
+
+* one FP64 operator apply, traction and single layer, to 1e-11;
+* the FP64 solve: the right-hand side is the oracle's b = -(1/2 I + K') rho0
+  (BIE/SphereSTKMobMat.cpp:196), the returned density x satisfies the oracle's
+  equation A x = b on the sampled spheres to the GMRES tolerance, and the
+  velocities are the oracle's rigid-body averages of S[rho0 + x]
+  (MobMat.cpp:253-285) to 1e-11;
+* the mixed form: velocities within 1e-7 of the FP64 form's (the bar for the
+  mixed form), and its density satisfies the oracle's equation as well.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+import spheresimulator_amd as S
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+SAMPLE = ((0, 8), (508, 516), (1016, 1024))
+
+
+def say(msg):
+    print("[case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def cfg2():
+    say("1024 spheres p=12: inputs and oracle operator")
+    c = S.configs.config2()
+    ref = O.Operator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+    coords, norms, w, off = ref.grid()
+    rows = np.concatenate([np.arange(3 * off[a], 3 * off[b]) for a, b in SAMPLE])
+    return dict(c=c, ref=ref, coords=coords, norms=norms, w=w, off=off, rows=rows)
+
+
+def sampled(ref, x, cId, cSL, cTrac, cLOP=0.0):
+    """The oracle's cId x + cSL S[x] + cTrac K'[x] + cLOP L[x] on the sampled spheres."""
+    out = np.zeros(ref.dof)
+    for a, b in SAMPLE:
+        ref.apply_p2p_spheres(x, a, b, out, cId, cSL, cTrac)
+    if cLOP:
+        ref.apply_lop(x, out, cLOP)
+    return out
+
+
+def rho0_of(g):
+    """rho0 = F/(4 pi R^2) + 3/(8 pi R^4) T x (R n)   (MobMat.cpp:174-192)."""
+    c = g["c"]
+    F = c["force"].reshape(-1, 6)
+    sph = np.repeat(np.arange(len(c["radius"])), np.diff(g["off"]))
+    R = c["radius"][sph][:, None]
+    return (F[sph, :3] / (4 * np.pi * R ** 2) +
+            3.0 / (8 * np.pi * R ** 4) * np.cross(F[sph, 3:], R * g["norms"])).ravel()
+
+
+def rigid_averages(g, u, spheres):
+    """V = sum w u / (4 pi R^2), Omega = 3 sum w (R n) x u / (8 pi R^4) per sphere (mu = 1)."""
+    out = []
+    for i in spheres:
+        a, b = g["off"][i], g["off"][i + 1]
+        R = g["c"]["radius"][i]
+        ui = u[3 * a:3 * b].reshape(-1, 3)
+        w = g["w"][a:b, None]
+        V = (w * ui).sum(0) / (4 * np.pi * R ** 2)
+        Om = 3.0 * (w * np.cross(R * g["norms"][a:b], ui)).sum(0) / (8 * np.pi * R ** 4)
+        out.append(np.concatenate([V, Om]))
+    return np.array(out)
+
+
+@pytest.fixture(scope="module")
+def fp64_solve(cfg2):
+    say("1024 spheres p=12: FP64 solve")
+    c = cfg2["c"]
+    M = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], precision=S.PREC_FP64)
+    v, info = M.solveMob(c["force"])
+    return dict(v=v, info=info, rho=M.getDensitySolution(), x=M.get_unknown())
+
+
+def test_fullsize_operator_apply_parity(cfg2):
+    c, ref, rows = cfg2["c"], cfg2["ref"], cfg2["rows"]
+    say("1024 spheres p=12: operator applies on sampled spheres")
+    gpu = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+    x = np.random.default_rng(21).standard_normal(ref.dof)
+    got = gpu.apply(x)
+    want = sampled(ref, x, 0.5, 0.0, 1.0, 1.0)
+    err = rel(got[rows], want[rows])
+    print("A x: rel err %.3e" % err)
+    assert err < 1e-11
+    got = gpu.applyP2POP(x, np.zeros(ref.dof), 0.0, 1.0, 0.0)
+    want = sampled(ref, x, 0.0, 1.0, 0.0)
+    err = rel(got[rows], want[rows])
+    print("S x: rel err %.3e" % err)
+    assert err < 1e-11
+
+
+def test_fullsize_fp64_solve_satisfies_oracle_equation(cfg2, fp64_solve):
+    ref, rows = cfg2["ref"], cfg2["rows"]
+    s = fp64_solve
+    print("FP64 solve: iters %d relres %.3e" % (s["info"]["iters"], s["info"]["relres"]))
+    assert s["info"]["converged"] == 1 and s["info"]["relres"] <= 1e-8
+    rho0 = rho0_of(cfg2)
+    assert rel(s["rho"] - s["x"], rho0) < 1e-14
+    b = sampled(ref, rho0, -0.5, 0.0, -1.0)
+    r = b[rows] - sampled(ref, s["x"], 0.5, 0.0, 1.0, 1.0)[rows]
+    res = float(np.linalg.norm(r) / np.linalg.norm(b[rows]))
+    print("oracle residual on sampled spheres %.3e" % res)
+    assert res < 1e-7
+    u = sampled(ref, s["rho"], 0.0, 1.0, 0.0)
+    spheres = np.concatenate([np.arange(a, b) for a, b in SAMPLE])
+    want = rigid_averages(cfg2, u, spheres)
+    got = s["v"].reshape(-1, 6)[spheres]
+    eV, eW = rel(got[:, :3], want[:, :3]), rel(got[:, 3:], want[:, 3:])
+    print("velocities from the oracle's S[rho]: rel err V %.3e Omega %.3e" % (eV, eW))
+    assert eV < 1e-11 and eW < 1e-11
+
+
+def test_fullsize_mixed_solve_against_fp64_and_oracle(cfg2, fp64_solve):
+    c, ref, rows = cfg2["c"], cfg2["ref"], cfg2["rows"]
+    say("1024 spheres p=12: mixed solve")
+    M = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], precision=S.PREC_MIXED)
+    v, info = M.solveMob(c["force"])
+    print("mixed solve: outer %d inner iters %d relres %.3e" % (info["outer"], info["iters"], info["relres"]))
+    assert info["converged"] == 1 and info["relres"] <= 1e-8
+    v64 = fp64_solve["v"].reshape(-1, 6)
+    vm = v.reshape(-1, 6)
+    eV, eW = rel(vm[:, :3], v64[:, :3]), rel(vm[:, 3:], v64[:, 3:])
+    print("mixed vs FP64: rel err V %.3e Omega %.3e" % (eV, eW))
+    assert eV < 1e-7 and eW < 1e-7
+    x = M.get_unknown()
+    b = sampled(ref, rho0_of(cfg2), -0.5, 0.0, -1.0)
+    r = b[rows] - sampled(ref, x, 0.5, 0.0, 1.0, 1.0)[rows]
+    res = float(np.linalg.norm(r) / np.linalg.norm(b[rows]))
+    print("oracle residual of the mixed density on sampled spheres %.3e" % res)
+    assert res < 1e-7

@@ -133,3 +133,30 @@ def test_operator_p2p_decomposition(pos_initial):
     ref = op.apply_lop(x, ref, 1.0)
     y = op.apply(x, y0.copy(), alpha=2.0, beta=-0.5)
     assert rel(y, 2.0 * ref - 0.5 * y0) < 1e-14
+
+
+def test_operator_p2p_on_sphere_ranges(pos_initial):
+    """The sphere-range apply (used at full problem sizes on sampled spheres)
+    equals the full apply on the points of its spheres, bit for bit, and leaves
+    every other entry untouched; empty and clipped ranges are no-ops or clipped."""
+    pos, rad = pos_initial
+    n = 30
+    rng = np.random.default_rng(4)
+    quat = np.array([quat_of(rng.standard_normal(3), rng.uniform(0, 3)) for _ in range(n)])
+    op = O.Operator(pos[:n], rad[:n], quat, [5, 6, 7] * (n // 3))
+    _, _, _, off = op.grid()
+    x = rng.standard_normal(op.dof)
+    for coeffs in ((0.5, 0.0, 1.0), (0.0, 1.0, 0.0), (0.3, 0.7, -0.4)):
+        full = op.apply_p2p(x, None, *coeffs)
+        for T0, T1 in ((0, 4), (13, 14), (25, 30), (0, n)):
+            a, b = 3 * off[T0], 3 * off[T1]
+            got = op.apply_p2p_spheres(x, T0, T1, None, *coeffs)
+            assert np.array_equal(got[a:b], full[a:b])
+            assert not got[:a].any() and not got[b:].any()
+            seed = rng.standard_normal(op.dof)
+            acc = op.apply_p2p_spheres(x, T0, T1, seed.copy(), *coeffs)
+            assert np.abs(acc[a:b] - seed[a:b] - full[a:b]).max() <= 1e-13 * np.abs(full).max()
+            assert np.array_equal(acc[:a], seed[:a]) and np.array_equal(acc[b:], seed[b:])
+    assert np.array_equal(op.apply_p2p_spheres(x, 5, 5, np.ones(op.dof), 0.5, 0.0, 1.0), np.ones(op.dof))
+    clip = op.apply_p2p_spheres(x, n - 2, n + 10, None, 0.5, 0.0, 1.0)
+    assert np.array_equal(clip, op.apply_p2p_spheres(x, n - 2, n, None, 0.5, 0.0, 1.0))
