@@ -113,18 +113,19 @@ int sphstk_operator_apply_device(sphstk_operator *op, const double *dX, double *
 /* The FP32 inner operator of the mixed-precision solve, Y := beta Y + alpha A32 X
  * (one column).  No counterpart in the reference: A32 has the far field in FP32
  * and, unless SPHSTK_FLAG_FP32_DIRECT was given at creation, multi-level (sphere
- * pairs farther apart than 4 (R_s + R_t) interact through coarser grids of order
- * p/2+1, max(5, p/3+1), 4 and 3 from 4, 8, 14 and 22 (R_s + R_t) on; orders
- * below 10 keep their own grid at the first level); self and near terms stay FP64. */
+ * pairs farther apart than 3 (R_s + R_t) interact through coarser grids of order
+ * p/2+1, max(5, p/3+1), 4 and 3 from 3, 6, 11 and 18 (R_s + R_t) on; orders
+ * below 10 keep their own grid at the first level); self and near terms stay FP64;
+ * relative error about 4e-6 against A64. */
 int sphstk_operator_apply_inner(sphstk_operator *op, const double *X, double *Y, double alpha, double beta);
 
 /* The FP64 outer operator of the mixed-precision solve (its right-hand side and
  * refinement residuals), Y := beta Y + alpha A64ml X (one column).  No
  * counterpart in the reference (whose far field is an FMM to a set tolerance):
- * FP64 pair sums with sphere pairs farther apart than 8 (R_s + R_t)
+ * FP64 pair sums with sphere pairs farther apart than 7 (R_s + R_t)
  * interacting through coarser grids of order min(p, 9), min(p, 7), min(p, 6)
- * and min(p, 5) from 8, 16, 32 and 64 (R_s + R_t) on; self and near terms as
- * A64; relative error about 1e-11 against A64. */
+ * and min(p, 5) from 7, 14, 28 and 56 (R_s + R_t) on; self and near terms as
+ * A64; relative error about 4e-11 against A64. */
 int sphstk_operator_apply_outer(sphstk_operator *op, const double *X, double *Y, double alpha, double beta);
 
 /* applyP2POP: out += cId in + cSL S[in] + cTrac K'[in]  (cpp:262-485; hpp:204-206). */

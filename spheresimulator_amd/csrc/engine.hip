@@ -286,11 +286,11 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     if (const char *v = std::getenv("SPHSTK_FAR_TPL")) tpl = std::atoi(v);
     if (const char *v = std::getenv("SPHSTK_FAR_SPLIT")) split = std::atoi(v);
     // FP32 inner operator: orders from level_order(); FP64 outer applies: fine
-    // below 8 (R_s + R_t), then orders 9, 7, 6 from 8, 16, 32 (R_s + R_t).
-    ls32_.alo[0] = 4.0, ls32_.alo[1] = 8.0, ls32_.alo[2] = 14.0, ls32_.alo[3] = 22.0;
+    // below 7 (R_s + R_t), then orders 9, 7, 6, 5 from 7, 14, 28, 56 (R_s + R_t).
+    ls32_.alo[0] = 3.0, ls32_.alo[1] = 6.0, ls32_.alo[2] = 11.0, ls32_.alo[3] = 18.0;
     ls32_.order[0] = ls32_.order[1] = ls32_.order[2] = ls32_.order[3] = -1;
     ls32_.fp64 = false;
-    ls64_.alo[0] = 8.0, ls64_.alo[1] = 16.0, ls64_.alo[2] = 32.0, ls64_.alo[3] = 64.0;
+    ls64_.alo[0] = 7.0, ls64_.alo[1] = 14.0, ls64_.alo[2] = 28.0, ls64_.alo[3] = 56.0;
     ls64_.order[0] = 9, ls64_.order[1] = 7, ls64_.order[2] = 6, ls64_.order[3] = 5;
     ls64_.fp64 = true;
     if (const char *v = std::getenv("SPHSTK_LEVEL_ALPHA"))
@@ -820,15 +820,17 @@ static std::vector<Chunk> make_chunks(int nsusp, const std::vector<int64_t> &sus
 
 // Multi-level far field of the FP32 inner operator.  With s = |c_S - c_T| /
 // (R_S + R_T), a far pair (s >= 1.5) interacts
-//   s < 4        point to point on the fine grids ("mid"),
-//   4 <= s < 8   between grids of order p/2+1            (level 0),
-//   8 <= s < 14  between grids of order max(5, p/3+1)    (level 1),
-//   s >= 14      between grids of order 4                (level 2),
+//   s < 3        point to point on the fine grids ("mid"),
+//   3 <= s < 6   between grids of order p/2+1 (p < 10: p) (level 0),
+//   6 <= s < 11  between grids of order max(5, p/3+1)    (level 1),
+//   11 <= s < 18 between grids of order 4                (level 2),
+//   s >= 18      between grids of order 3                (level 3),
 // (orders capped by p) through the degree-truncated density of the source
 // (Resample::down) with the target-grid field interpolated back to the fine grid
-// (Resample::up).  The truncated degrees of each level contribute below ~1e-6
-// relative: the field of degree-n density decays like (1/(2s))^(n+2) and its
-// angular content on the target like (1/(2s-1))^(n+1).  Level point sets are
+// (Resample::up).  The field of degree-n density decays like (1/(2s))^(n+2) and
+// its angular content on the target like (1/(2s-1))^(n+1); these thresholds put
+// the operator's error near 4e-6 relative (1024 spheres, p=12), well inside what
+// two refinement steps against the FP64 residual need.  Level point sets are
 // stored in Morton order of the sphere centres (per suspension) so that the
 // spatially compact exclusion sets of the last level are few index ranges.
 namespace {

@@ -224,12 +224,14 @@ def test_set_density_warm_start():
     assert i3["iters"] == i1["iters"] and np.array_equal(v3, v1)
 
 
-@pytest.mark.parametrize("p,direct,tol", [(8, True, 2e-6), (8, False, 2e-6), (12, False, 1e-6)])
+@pytest.mark.parametrize("p,direct,tol", [(8, True, 2e-6), (8, False, 2e-6), (12, False, 1e-5)])
 def test_inner_fp32_operator_accuracy(p, direct, tol):
     """The FP32 inner operator approximates the FP64 operator on a white band-
     limited density (the projection of a random field).  Direct FP32 far field:
-    FP32 rounding only.  Multi-level far field (pairs beyond 4 (R_s + R_t) through
-    coarser grids, order p/2+1 first): the truncated degrees decay like (1/8)^(p/2+2)."""
+    FP32 rounding only.  Multi-level far field (pairs beyond 3 (R_s + R_t) through
+    coarser grids, order p/2+1 first): the truncated degrees decay like (1/6)^(p/2+2),
+    about 4e-6 of the operator at p=12; two refinement steps against the FP64
+    residual reach 1e-8 with an inner operator up to ~1e-5 off."""
     say("inner FP32 operator (p=%d direct=%s)" % (p, direct))
     c = S.configs.make_suspension(96 if p == 8 else 256, p, 31, vf=0.1, min_gap=0.2)
     gpu = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0,
@@ -269,7 +271,7 @@ def test_far_traction_mfma_path_matches():
 @pytest.mark.parametrize("p", [10, 12])
 def test_outer_fp64_operator_accuracy(p):
     """The FP64 multi-level outer operator of the mixed form (pairs beyond
-    8 (R_s + R_t) through grids of order min(p, 9/7/6)) matches the FP64 direct
+    7 (R_s + R_t) through grids of order min(p, 9/7/6/5)) matches the FP64 direct
     operator on a band-limited density and on a white one (no band limit), to
     far below the 1e-8 GMRES tolerance the refinement residual serves.  The
     dilute suspension (vf 0.02) makes the levels pay, so they are in use."""
@@ -277,7 +279,7 @@ def test_outer_fp64_operator_accuracy(p):
     c = S.configs.make_suspension(256, p, 33, vf=0.02, min_gap=0.2)
     gpu = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
     white = np.random.default_rng(9).standard_normal(gpu.n_dof)
-    for name, x, tol in (("band-limited", gpu.projectNullSpace(white.copy()), 2e-11), ("white", white, 1e-10)):
+    for name, x, tol in (("band-limited", gpu.projectNullSpace(white.copy()), 1e-10), ("white", white, 1e-10)):
         err = rel(gpu.apply_outer(x), gpu.apply(x))
         print("outer %s rel err %.3e" % (name, err))
         assert err < tol
