@@ -123,9 +123,9 @@ int sphstk_operator_apply_inner(sphstk_operator *op, const double *X, double *Y,
  * refinement residuals), Y := beta Y + alpha A64ml X (one column).  No
  * counterpart in the reference (whose far field is an FMM to a set tolerance):
  * FP64 pair sums with sphere pairs farther apart than 7 (R_s + R_t)
- * interacting through coarser grids of order min(p, 9), min(p, 7), min(p, 6)
- * and min(p, 5) from 7, 14, 28 and 56 (R_s + R_t) on; self and near terms as
- * A64; relative error about 4e-11 against A64. */
+ * interacting through coarser grids of order min(p, 8), min(p, 6), min(p, 5)
+ * and min(p, 4) from 7, 14, 28 and 56 (R_s + R_t) on; self and near terms as
+ * A64; relative error about 1e-9 against A64. */
 int sphstk_operator_apply_outer(sphstk_operator *op, const double *X, double *Y, double alpha, double beta);
 
 /* applyP2POP: out += cId in + cSL S[in] + cTrac K'[in]  (cpp:262-485; hpp:204-206). */

@@ -286,12 +286,12 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     if (const char *v = std::getenv("SPHSTK_FAR_TPL")) tpl = std::atoi(v);
     if (const char *v = std::getenv("SPHSTK_FAR_SPLIT")) split = std::atoi(v);
     // FP32 inner operator: orders from level_order(); FP64 outer applies: fine
-    // below 7 (R_s + R_t), then orders 9, 7, 6, 5 from 7, 14, 28, 56 (R_s + R_t).
+    // below 7 (R_s + R_t), then orders 8, 6, 5, 4 from 7, 14, 28, 56 (R_s + R_t).
     ls32_.alo[0] = 3.0, ls32_.alo[1] = 6.0, ls32_.alo[2] = 11.0, ls32_.alo[3] = 18.0;
     ls32_.order[0] = ls32_.order[1] = ls32_.order[2] = ls32_.order[3] = -1;
     ls32_.fp64 = false;
     ls64_.alo[0] = 7.0, ls64_.alo[1] = 14.0, ls64_.alo[2] = 28.0, ls64_.alo[3] = 56.0;
-    ls64_.order[0] = 9, ls64_.order[1] = 7, ls64_.order[2] = 6, ls64_.order[3] = 5;
+    ls64_.order[0] = 8, ls64_.order[1] = 6, ls64_.order[2] = 5, ls64_.order[3] = 4;
     ls64_.fp64 = true;
     if (const char *v = std::getenv("SPHSTK_LEVEL_ALPHA"))
         if (std::sscanf(v, "%lf,%lf,%lf,%lf", &ls32_.alo[0], &ls32_.alo[1], &ls32_.alo[2], &ls32_.alo[3]) != 4)

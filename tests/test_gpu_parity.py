@@ -271,15 +271,16 @@ def test_far_traction_mfma_path_matches():
 @pytest.mark.parametrize("p", [10, 12])
 def test_outer_fp64_operator_accuracy(p):
     """The FP64 multi-level outer operator of the mixed form (pairs beyond
-    7 (R_s + R_t) through grids of order min(p, 9/7/6/5)) matches the FP64 direct
+    7 (R_s + R_t) through grids of order min(p, 8/6/5/4)) matches the FP64 direct
     operator on a band-limited density and on a white one (no band limit), to
-    far below the 1e-8 GMRES tolerance the refinement residual serves.  The
+    far below the 1e-8 GMRES tolerance the refinement residual serves (about
+    8e-10 at 1024 spheres, p=12).  The
     dilute suspension (vf 0.02) makes the levels pay, so they are in use."""
     say("outer FP64 multi-level operator (p=%d)" % p)
     c = S.configs.make_suspension(256, p, 33, vf=0.02, min_gap=0.2)
     gpu = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
     white = np.random.default_rng(9).standard_normal(gpu.n_dof)
-    for name, x, tol in (("band-limited", gpu.projectNullSpace(white.copy()), 1e-10), ("white", white, 1e-10)):
+    for name, x, tol in (("band-limited", gpu.projectNullSpace(white.copy()), 2e-9), ("white", white, 2e-9)):
         err = rel(gpu.apply_outer(x), gpu.apply(x))
         print("outer %s rel err %.3e" % (name, err))
         assert err < tol
