@@ -293,9 +293,13 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     ls64_.alo[0] = 7.0, ls64_.alo[1] = 14.0, ls64_.alo[2] = 28.0, ls64_.alo[3] = 56.0;
     ls64_.order[0] = 8, ls64_.order[1] = 6, ls64_.order[2] = 5, ls64_.order[3] = 4;
     ls64_.fp64 = true;
-    if (const char *v = std::getenv("SPHSTK_LEVEL_ALPHA"))
-        if (std::sscanf(v, "%lf,%lf,%lf,%lf", &ls32_.alo[0], &ls32_.alo[1], &ls32_.alo[2], &ls32_.alo[3]) != 4)
-            throw Error(-1, "SPHSTK_LEVEL_ALPHA: a0,a1,a2,a3");
+    // separations, optionally followed by orders (-1: the order-dependent default)
+    if (const char *v = std::getenv("SPHSTK_LEVEL_ALPHA")) {
+        const int got = std::sscanf(v, "%lf,%lf,%lf,%lf:%d,%d,%d,%d", &ls32_.alo[0], &ls32_.alo[1], &ls32_.alo[2],
+                                    &ls32_.alo[3], &ls32_.order[0], &ls32_.order[1], &ls32_.order[2],
+                                    &ls32_.order[3]);
+        if (got != 4 && got != 8) throw Error(-1, "SPHSTK_LEVEL_ALPHA: a0,a1,a2,a3[:p0,p1,p2,p3]");
+    }
     if (const char *v = std::getenv("SPHSTK_OUTER_LEVELS"))
         if (std::sscanf(v, "%lf,%lf,%lf,%lf:%d,%d,%d,%d", &ls64_.alo[0], &ls64_.alo[1], &ls64_.alo[2], &ls64_.alo[3],
                         &ls64_.order[0], &ls64_.order[1], &ls64_.order[2], &ls64_.order[3]) != 8)
