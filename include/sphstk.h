@@ -63,7 +63,7 @@ typedef struct {
     int restart;        /* Num Blocks (yaml :7); 30 */
     int max_iters;      /* Maximum Iterations (yaml :6) */
     int explicit_check; /* 1: confirm convergence with ||b - A x|| (Belos explicit residual test) */
-    double inner_tol;   /* mixed precision only: inner FP32 GMRES tolerance (1e-5) */
+    double inner_tol;   /* mixed precision only: inner FP32 GMRES tolerance (3e-5) */
     int max_refine;     /* mixed precision only: maximum outer refinement steps */
 } sphstk_solver_params;
 

@@ -198,7 +198,7 @@ class SphereSTKSHOperator:
         return dict(tpl=t.value, split=s.value, inner_tpl=i.value)
 
 
-def solver_params(tol=1e-8, restart=30, max_iters=1000, explicit_check=1, inner_tol=1e-5, max_refine=10):
+def solver_params(tol=1e-8, restart=30, max_iters=1000, explicit_check=1, inner_tol=3e-5, max_refine=10):
     return SolverParams(tol, restart, max_iters, explicit_check, inner_tol, max_refine)
 
 

@@ -116,7 +116,7 @@ struct SolverParams {
     int restart = 30;
     int max_iters = 1000;
     int explicit_check = 1;
-    double inner_tol = 1e-5;
+    double inner_tol = 3e-5;
     int max_refine = 10;
 };
 
