@@ -244,6 +244,37 @@ def test_inner_fp32_operator_accuracy(p, direct, tol):
     assert err < tol
 
 
+def test_inner_level_orders_from_environment(monkeypatch):
+    """SPHSTK_LEVEL_ALPHA with explicit FP32 level orders: orders equal to the
+    defaults (p=8: 8, 5, 4, 3; -1 keeps a default) give the default inner
+    operator bit for bit, lower orders on the far levels change it slightly,
+    and a malformed value makes creation fail with an error."""
+    say("inner level orders from SPHSTK_LEVEL_ALPHA")
+    c = S.configs.make_suspension(96, 8, 31, vf=0.02, min_gap=0.2)
+
+    def inner(x):
+        op = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+        return op.apply_inner(x)
+
+    monkeypatch.delenv("SPHSTK_LEVEL_ALPHA", raising=False)
+    op = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+    x = np.random.default_rng(5).standard_normal(op.n_dof)
+    ref, y64 = op.apply_inner(x), op.apply(x)
+    del op
+    for v in ("3,6,11,18:-1,-1,-1,-1", "3,6,11,18:8,5,4,3", "3,6,11,18"):
+        monkeypatch.setenv("SPHSTK_LEVEL_ALPHA", v)
+        assert np.array_equal(inner(x), ref), v
+    monkeypatch.setenv("SPHSTK_LEVEL_ALPHA", "3,6,11,18:-1,-1,3,2")
+    low = inner(x)
+    print("lower far-level orders: change %.3e, error vs FP64 %.3e (default %.3e)" %
+          (rel(low, ref), rel(low, y64), rel(ref, y64)))
+    assert 0 < rel(low, ref) < 1e-3
+    for bad in ("3,6,11", "3,6,11,18:1,2", "3,6,11,18:-1,-1,-1,99"):
+        monkeypatch.setenv("SPHSTK_LEVEL_ALPHA", bad)
+        with pytest.raises(S.SphstkError):
+            inner(x)
+
+
 def test_far_traction_mfma_path_matches():
     """The opt-in matrix-core FP64 traction far field (SPHSTK_FAR_MFMA=1:
     sphere-aligned work items, dot products on v_mfma_f64_16x16x4f64) agrees
