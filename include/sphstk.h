@@ -53,6 +53,8 @@ extern "C" {
 #define SPHSTK_FLAG_FP32_DIRECT 2 /* FP32 inner operator: direct far field (no multi-level coarse grids) */
 #define SPHSTK_FLAG_AUTOTUNE 4    /* time candidate far-field tile shapes at set-up and keep the fastest */
 #define SPHSTK_FLAG_GRAPH 8       /* replay the FP32 inner operator's apply from a captured HIP graph */
+#define SPHSTK_FLAG_INNER_PRECOND 16 /* mixed form: right-precondition the inner solve by the isolated-sphere
+                                        inverse (sphstk_operator_apply_self_inverse) */
 
 typedef struct sphstk_operator sphstk_operator;
 typedef struct sphstk_mobility sphstk_mobility;
@@ -72,7 +74,11 @@ typedef struct {
     int outer;              /* refinement steps (mixed precision), 1 for FP64 */
     int converged;          /* the reference ignores Belos' status (MobMat.cpp:236); reported here */
     int shortcut;           /* ||b|| < 1e-5 shortcut taken (MobMat.cpp:216-219) */
-    double relres;          /* final ||b - A x||/||b|| (explicit when computed, else implicit) */
+    double relres;          /* final ||b - A x||/||b|| (explicit when computed, else implicit).  FP64
+                               form: A is the FP64 operator with the direct far-field sum.  Mixed form:
+                               A is the multi-level FP64 operator of sphstk_operator_apply_outer (about
+                               1e-9 from the direct-sum operator), b its right-hand side with distant
+                               spheres' rho0 in closed form */
     int applies_fp64;       /* FP64 operator applies, including the RHS and final single-layer applies */
     int applies_fp32;       /* FP32 operator applies (mixed precision) */
 } sphstk_solve_info;

@@ -21,6 +21,7 @@ FLAG_TIMING = 1
 FLAG_FP32_DIRECT = 2
 FLAG_AUTOTUNE = 4
 FLAG_GRAPH = 8
+FLAG_INNER_PRECOND = 16
 
 
 def _d(a):
@@ -31,6 +32,15 @@ def _f64(a, n=None, name="array"):
     a = np.ascontiguousarray(a, dtype=np.float64).ravel()
     if n is not None and a.size != n:
         raise ValueError("%s has %d entries, expected %d" % (name, a.size, n))
+    return a
+
+
+def _out(a, n, name):
+    """An output array the library writes n doubles into: float64, C-contiguous, n entries."""
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not a.flags.c_contiguous or a.size != n:
+        raise ValueError("%s must be a C-contiguous float64 array of %d entries" % (name, n))
+    if not a.flags.writeable:
+        raise ValueError("%s must be writeable" % name)
     return a
 
 
@@ -85,42 +95,34 @@ class SphereSTKSHOperator:
         if X.size % self.n_dof:
             raise ValueError("X length is not a multiple of the DOF count")
         ncol = X.size // self.n_dof
-        Y = np.zeros(X.size) if Y is None else Y
-        if Y.size != X.size or Y.dtype != np.float64 or not Y.flags.c_contiguous:
-            raise ValueError("Y must be a contiguous float64 array of the size of X")
+        Y = np.zeros(X.size) if Y is None else _out(Y, X.size, "Y")
         check(load().sphstk_operator_apply(self._h, _d(X), _d(Y), ncol, alpha, beta))
         return Y
 
     def apply_inner(self, X, Y=None, alpha=1.0, beta=0.0):
         """The FP32 inner operator of the mixed-precision solve (one column)."""
         X = _f64(X, self.n_dof, "X")
-        Y = np.zeros(self.n_dof) if Y is None else Y
-        if Y.size != self.n_dof or Y.dtype != np.float64 or not Y.flags.c_contiguous:
-            raise ValueError("Y must be a contiguous float64 array of length n_dof")
+        Y = np.zeros(self.n_dof) if Y is None else _out(Y, self.n_dof, "Y")
         check(load().sphstk_operator_apply_inner(self._h, _d(X), _d(Y), alpha, beta))
         return Y
 
     def apply_outer(self, X, Y=None, alpha=1.0, beta=0.0):
         """The FP64 multi-level outer operator of the mixed-precision solve (one column)."""
         X = _f64(X, self.n_dof, "X")
-        Y = np.zeros(self.n_dof) if Y is None else Y
-        if Y.size != self.n_dof or Y.dtype != np.float64 or not Y.flags.c_contiguous:
-            raise ValueError("Y must be a contiguous float64 array of length n_dof")
+        Y = np.zeros(self.n_dof) if Y is None else _out(Y, self.n_dof, "Y")
         check(load().sphstk_operator_apply_outer(self._h, _d(X), _d(Y), alpha, beta))
         return Y
 
     def applyP2POP(self, inp, out, cId, cSL, cTrac):
         """out += cId in + cSL S[in] + cTrac K'[in]  (accumulates, as the reference)."""
         inp = _f64(inp, self.n_dof, "in")
-        if out.size != self.n_dof or out.dtype != np.float64:
-            raise ValueError("out must be a float64 array of length n_dof")
+        _out(out, self.n_dof, "out")
         check(load().sphstk_operator_apply_p2p(self._h, _d(inp), _d(out), cId, cSL, cTrac))
         return out
 
     def applyLOP(self, inp, out, cLOP):
         inp = _f64(inp, self.n_dof, "in")
-        if out.size != self.n_dof or out.dtype != np.float64:
-            raise ValueError("out must be a float64 array of length n_dof")
+        _out(out, self.n_dof, "out")
         check(load().sphstk_operator_apply_lop(self._h, _d(inp), _d(out), cLOP))
         return out
 
@@ -133,8 +135,7 @@ class SphereSTKSHOperator:
         return y
 
     def projectNullSpace(self, inout):
-        if inout.size != self.n_dof or inout.dtype != np.float64:
-            raise ValueError("inout must be a float64 array of length n_dof")
+        _out(inout, self.n_dof, "inout")
         check(load().sphstk_operator_project_nullspace(self._h, _d(inout)))
         return inout
 
@@ -206,7 +207,7 @@ class SphereSTKMobMat:
     """Force/torque (6N) -> velocity/angular velocity (6N) through A = 0.5 I + K' + L."""
 
     def __init__(self, pos, radius, quat, order, viscosity=1.0, counts=None, precision=PREC_FP64, params=None,
-                 timing=False, fp32_direct=False, autotune=False, graph=False):
+                 timing=False, fp32_direct=False, autotune=False, graph=False, inner_precond=False):
         L = load()
         s = _Spheres(pos, radius, quat, order, counts)
         self.n_spheres = s.n
@@ -219,9 +220,19 @@ class SphereSTKMobMat:
                                                (FLAG_TIMING if timing else 0) |
                                                (FLAG_FP32_DIRECT if fp32_direct else 0) |
                                                (FLAG_AUTOTUNE if autotune else 0) |
-                                               (FLAG_GRAPH if graph else 0), ctypes.byref(h)))
+                                               (FLAG_GRAPH if graph else 0) |
+                                               (FLAG_INNER_PRECOND if inner_precond else 0), ctypes.byref(h)))
         self._h = h
-        self.op = SphereSTKSHOperator(None, None, None, None, _handle=L.sphstk_mobility_operator(h), _owner=self)
+        self.n_dof = self.op.n_dof
+
+    @property
+    def op(self):
+        """The operator A = 0.5 I + K' + L owned by this mobility.  A view made on
+        each access: it keeps the mobility alive while it is referenced, and the
+        mobility holds no reference to it, so dropping the mobility frees its
+        device memory at once."""
+        return SphereSTKSHOperator(None, None, None, None, _handle=load().sphstk_mobility_operator(self._h),
+                                   _owner=self)
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -252,26 +263,26 @@ class SphereSTKMobMat:
         n6 = 6 * self.n_spheres
         if X.size % n6:
             raise ValueError("X length is not a multiple of 6 N")
-        Y = np.zeros(X.size) if Y is None else Y
+        Y = np.zeros(X.size) if Y is None else _out(Y, X.size, "Y")
         check(load().sphstk_mobility_apply(self._h, _d(X), _d(Y), X.size // n6, alpha, beta))
         return Y
 
     def getDensitySolution(self):
-        rho = np.zeros(self.op.n_dof)
+        rho = np.zeros(self.n_dof)
         check(load().sphstk_mobility_get_density(self._h, _d(rho)))
         return rho
 
     def get_unknown(self):
-        x = np.zeros(self.op.n_dof)
+        x = np.zeros(self.n_dof)
         check(load().sphstk_mobility_get_unknown(self._h, _d(x)))
         return x
 
     def set_unknown(self, x):
-        x = _f64(x, self.op.n_dof, "x")
+        x = _f64(x, self.n_dof, "x")
         check(load().sphstk_mobility_set_unknown(self._h, _d(x)))
 
     def set_density(self, rho):
         """Warm start from a stored density, x = projectNullSpace(rho), as the
         reference's constructor does with the layer's gridValue (MobMat.cpp:67-93)."""
-        rho = _f64(rho, self.op.n_dof, "rho")
+        rho = _f64(rho, self.n_dof, "rho")
         check(load().sphstk_mobility_set_density(self._h, _d(rho)))

@@ -73,7 +73,8 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     c_[1] = cSL;
     c_[2] = cTrac;
     c_[3] = cLOP;
-    SPHSTK_REQUIRE((flags & ~15) == 0, "unknown flags (known: TIMING 1, FP32_DIRECT 2, AUTOTUNE 4, GRAPH 8)");
+    SPHSTK_REQUIRE((flags & ~31) == 0,
+                   "unknown flags (known: TIMING 1, FP32_DIRECT 2, AUTOTUNE 4, GRAPH 8, INNER_PRECOND 16)");
     SPHSTK_REQUIRE(nsusp >= 1, "number of suspensions must be >= 1");
     SPHSTK_REQUIRE(ncopy >= 1 && ncopy <= 64 && nsusp % ncopy == 0, "copies must divide the suspensions");
     SPHSTK_REQUIRE(counts != nullptr, "sphere counts must not be NULL");
@@ -218,7 +219,7 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     }
 
     // device state
-    SPHSTK_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    stream_ = stream_owner_.create();
     d_pos_.upload(pos_);
     d_rad_.upload(rad_);
     d_quat_.upload(quat_);
@@ -283,8 +284,6 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     partial_.alloc(std::max<size_t>(ch.size(), 1));
 
     int tpl = 1, split = 0;
-    if (const char *v = std::getenv("SPHSTK_FAR_TPL")) tpl = std::atoi(v);
-    if (const char *v = std::getenv("SPHSTK_FAR_SPLIT")) split = std::atoi(v);
     // FP32 inner operator: orders from level_order(); FP64 outer applies: fine
     // below 7 (R_s + R_t), then orders 8, 6, 5, 4 from 7, 14, 28, 56 (R_s + R_t).
     ls32_.alo[0] = 3.0, ls32_.alo[1] = 6.0, ls32_.alo[2] = 11.0, ls32_.alo[3] = 18.0;
@@ -300,10 +299,6 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
                                     &ls32_.order[3]);
         if (got != 4 && got != 8) throw Error(-1, "SPHSTK_LEVEL_ALPHA: a0,a1,a2,a3[:p0,p1,p2,p3]");
     }
-    if (const char *v = std::getenv("SPHSTK_OUTER_LEVELS"))
-        if (std::sscanf(v, "%lf,%lf,%lf,%lf:%d,%d,%d,%d", &ls64_.alo[0], &ls64_.alo[1], &ls64_.alo[2], &ls64_.alo[3],
-                        &ls64_.order[0], &ls64_.order[1], &ls64_.order[2], &ls64_.order[3]) != 8)
-            throw Error(-1, "SPHSTK_OUTER_LEVELS: a0,a1,a2,a3:p0,p1,p2,p3");
     for (const LevelSet *S : {&ls32_, &ls64_}) {
         SPHSTK_REQUIRE(S->alo[0] >= 1.5, "level separations must start at >= 1.5");
         for (int l = 1; l < kNumLevels; l++)
@@ -313,13 +308,6 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
                            "level orders must lie in [1, max order]");
     }
     multi_level_ = (flags_ & 2) == 0;
-    if (const char *v = std::getenv("SPHSTK_MULTI_LEVEL")) multi_level_ = std::atoi(v) != 0;
-    if (const char *v = std::getenv("SPHSTK_FAR_MFMA")) far_mfma_ = std::atoi(v) != 0;
-    if (ncopy_ > 1) {
-        far_mfma_ = false;
-        tpl = 1;
-    }
-    if (far_mfma_) tpl = 1;
     {
         PhaseTimer pt2("fine far-field lists");
         std::vector<std::vector<int>> excl(nsph_);
@@ -351,8 +339,7 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
             if (lists_fine_.split > 1) lists_fine_.part.alloc(size_t(lists_fine_.split) * 3 * npts_);
             srcm_.alloc(std::max<int64_t>(4 * (std::min(ncopy_, kMultiCols) + 1) * pt_off_[s0], 8));
         } else {
-            build_far_lists(lists_fine_, nsusp_, susp_sph_off_, pt_off_, pt_off_, runs_fine_, tpl, split, kWantItems,
-                            far_mfma_ ? pos_.data() : nullptr);
+            build_far_lists(lists_fine_, nsusp_, susp_sph_off_, pt_off_, pt_off_, runs_fine_, tpl, split, kWantItems);
         }
     }
     ls32_.tpl = ls64_.tpl = tpl;
@@ -361,8 +348,7 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     coef_.alloc(std::max<int64_t>(coef_off_[nsph_], 1));
     rec_.alloc(std::max<int64_t>(rec_off_[nsph_], 1));
     tmp_.alloc(std::max<int64_t>(3 * npts_, 1));
-    if ((flags_ & 4) && ncopy_ == 1 && std::getenv("SPHSTK_FAR_TPL") == nullptr &&
-        std::getenv("SPHSTK_FAR_SPLIT") == nullptr) {
+    if ((flags_ & 4) && ncopy_ == 1) {
         double budget = kTuneBudget;
         autotune_fine(budget);
     }
@@ -394,7 +380,7 @@ double Operator::time_far_pass(FarLists &L, int prec, const void *src, const dou
 }
 
 void Operator::autotune_fine(double &budget_s) {
-    if (nsph_ == 0 || far_mfma_) return;
+    if (nsph_ == 0) return;
     launch_fill(3 * npts_, 1.0, tmp_.p, stream_);
     launch_pack_src_f64(npts_, d_coords_.p, d_w_.p, tmp_.p, src64_.p, stream_);
     DevBuf<double> out;
@@ -513,7 +499,6 @@ Operator::~Operator() {
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
     }
-    if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 // Runs of the sorted sphere list `lst` within [s0, s1) (include) or of its
@@ -1014,15 +999,10 @@ void Operator::far_run(FarLists &L, int kind, int prec, const void *src, const d
                        const double *tnorms, const float *trg32, const Chunk *chunks, int nchunks, double coef,
                        double *out, const int *active) {
     if (L.pairs == 0.0) return; // e.g. a level that no sphere pair reaches
-    const bool mfma = kind == 0 && prec == 0 && L.aligned && far_mfma_;
     double *dst = L.split > 1 ? L.part.p : out;
     const int acc = L.split > 1 ? 0 : 1;
-    if (mfma)
-        launch_far_trac_mfma_f64((const double *)src, tcoords, tnorms, L.cen.p, L.ntrg, L.ngroups, L.order.p,
-                                 L.groups.p, L.iv.p, active, dst, coef, acc, stream_);
-    else
-        launch_far(kind, prec, L.tpl, src, tcoords, tnorms, trg32, L.ntrg, L.ngroups, L.order.p, L.groups.p, L.iv.p,
-                   active, dst, coef, acc, stream_, &L != &lists_fine_);
+    launch_far(kind, prec, L.tpl, src, tcoords, tnorms, trg32, L.ntrg, L.ngroups, L.order.p, L.groups.p, L.iv.p,
+               active, dst, coef, acc, stream_, &L != &lists_fine_);
     if (L.split > 1) launch_slot_reduce(out, L.part.p, L.split, 3 * L.ntrg, chunks, nchunks, active, stream_);
 }
 
@@ -1289,7 +1269,7 @@ void Operator::ensure_centre() {
 void Operator::prepare_mixed(bool outer_levels) {
     if (nsph_ == 0) return;
     if (multi_level_) {
-        if ((flags_ & 4) && std::getenv("SPHSTK_FAR_TPL") == nullptr) {
+        if (flags_ & 4) {
             double budget = kTuneBudget;
             autotune_inner(budget);
         }
@@ -1532,11 +1512,8 @@ void Operator::reset_stats() {
 Mobility::Mobility(int nsusp, const int *counts, const double *pos, const double *radius, const double *quat,
                    const int *order, double viscosity, const SolverParams &prm, int precision, int flags, int ncopy)
     : mu_(viscosity), prm_(prm), prec_(precision) {
-    if (const char *v = std::getenv("SPHSTK_OUTER_ML")) outer_prec_ = std::atoi(v) != 0 ? 2 : 0;
     inner_restart_ = std::max(prm.restart, 60);
-    if (const char *v = std::getenv("SPHSTK_INNER_PRECOND")) inner_precond_ = std::atoi(v) != 0;
-    if (const char *v = std::getenv("SPHSTK_INNER_RESTART")) inner_restart_ = std::atoi(v);
-    SPHSTK_REQUIRE(inner_restart_ >= 1 && inner_restart_ <= 1000, "SPHSTK_INNER_RESTART must lie in [1, 1000]");
+    inner_precond_ = (flags & 16) != 0;
     SPHSTK_REQUIRE(std::isfinite(viscosity) && viscosity > 0, "viscosity must be > 0");
     SPHSTK_REQUIRE(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (mixed)");
     if (const char *v = std::getenv("SPHSTK_VERBOSE")) verbose_ = std::atoi(v) != 0;

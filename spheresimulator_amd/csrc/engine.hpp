@@ -51,6 +51,22 @@ struct DevBuf {
     }
 };
 
+// Owning HIP stream (default priority, non-blocking).  A member of this type
+// releases the stream also when a constructor throws after creating it.
+struct StreamOwner {
+    hipStream_t s = nullptr;
+    StreamOwner() = default;
+    StreamOwner(const StreamOwner &) = delete;
+    StreamOwner &operator=(const StreamOwner &) = delete;
+    ~StreamOwner() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+    hipStream_t create() {
+        SPHSTK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        return s;
+    }
+};
+
 // Far-field work lists over one target point set (see build_far_lists).
 struct FarLists {
     DevBuf<FarGroup> groups;
@@ -211,6 +227,7 @@ class Operator {
     void far_levels(LevelSet &S, int kind, double coef, const double *in, double *out, const int *active,
                     hipEvent_t ev_a, hipEvent_t ev_b);
 
+    StreamOwner stream_owner_; // declared before every buffer, so destroyed after them
     int nsph_, nsusp_;
     int64_t npts_;
     static constexpr int kMultiCols = 8; // right-hand sides per shared far-field launch
@@ -245,7 +262,6 @@ class Operator {
     LevelSet ls32_, ls64_;
     bool multi_level_ = true, timing_done_ = false;
     int outer_pay_ = -1; // FP64 multi-level outer far field in use: -1 undecided, 0 no, 1 yes
-    bool far_mfma_ = false; // FP64 traction far field with the dot products on the matrix cores (opt-in)
 
     // captured inner-operator apply (SPHSTK_FLAG_GRAPH): g_in_ -> g_out_
     hipGraph_t g_inner_ = nullptr;

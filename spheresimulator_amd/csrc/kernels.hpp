@@ -64,11 +64,6 @@ void launch_far(int kind, int prec, int tpl, const void *src, const double *tcoo
                 const float *trg32, int64_t ntrg, int ngroups, const int *group_order, const FarGroup *groups,
                 const FarInterval *iv, const int *active, double *out, double coef, int accumulate, hipStream_t s,
                 bool levels = false);
-// FP64 traction far field on sphere-aligned lists with the dot products on the
-// matrix cores (far_mfma.hip); gcen holds each group's sphere centre.
-void launch_far_trac_mfma_f64(const double *src, const double *tcoords, const double *tnorms, const double *gcen,
-                              int64_t ntrg, int ngroups, const int *go, const FarGroup *gr, const FarInterval *iv,
-                              const int *active, double *out, double coef, int acc, hipStream_t s);
 
 // out_S (+)= R_S M_p R_S^-1 in_S between two point sets with per-sphere offsets
 void launch_resample(const DevGeom &g, const ResampleTabs &tab, const int64_t *in_off, const int64_t *out_off,

@@ -6,7 +6,9 @@ VTU files (Shexp::writeVTU) on the host; Brownian velocities
 (SRC/SphereSystem.cpp:323-360) and collision pairs
 (Sphere/CollisionSphere.hpp:51-90) on the GPU through the C ABI.
 Orientations are unit quaternions (w, x, y, z) as at the C ABI."""
+import base64
 import ctypes
+import struct
 
 import numpy as np
 
@@ -119,11 +121,41 @@ def _qrot(q, v):
     return v + w * t + np.cross(u, t)
 
 
+_VTK_TYPES = {"Float64": "<f8", "Int32": "<i4", "UInt8": "u1"}
+
+
 def _array(f, name, a, ncomp, vtype):
-    f.write('<DataArray type="%s" Name="%s" NumberOfComponents="%d" format="ascii">\n' % (vtype, name, ncomp))
-    fmt = "%d" if vtype.startswith(("Int", "UInt")) else "%.17g"
-    f.write(" ".join(fmt % x for x in np.asarray(a).ravel()))
+    """One inline binary DataArray of VTK XML: base64 of the byte count (UInt32,
+    the default header of file version 0.1) followed by base64 of the
+    little-endian values, the two encoded separately as VTK writes and reads
+    them (the reference's IOHelper::writeDataArrayBase64 output)."""
+    raw = np.ascontiguousarray(np.asarray(a).ravel(), dtype=_VTK_TYPES[vtype]).tobytes()
+    f.write('<DataArray type="%s" Name="%s" NumberOfComponents="%d" format="binary">\n' % (vtype, name, ncomp))
+    f.write(base64.b64encode(struct.pack("<I", len(raw))).decode("ascii"))
+    f.write(base64.b64encode(raw).decode("ascii"))
     f.write("\n</DataArray>\n")
+
+
+def read_vtu_arrays(path):
+    """The DataArrays of each Piece of a VTU file written by write_vtu (inline
+    base64 binary, or ascii): a list with one {name: numpy array} per Piece."""
+    import xml.etree.ElementTree as ET
+    out = []
+    for piece in ET.parse(path).getroot().find("UnstructuredGrid").findall("Piece"):
+        arrs = {}
+        for d in piece.iter("DataArray"):
+            dt = np.dtype(_VTK_TYPES[d.get("type")])
+            text = (d.text or "").strip()
+            if d.get("format") == "ascii":
+                arrs[d.get("Name")] = np.array(text.split(), dtype=dt)
+                continue
+            nbytes = struct.unpack("<I", base64.b64decode(text[:8]))[0]
+            data = base64.b64decode(text[8:])
+            if len(data) != nbytes:
+                raise ValueError("DataArray %s: %d bytes, header says %d" % (d.get("Name"), len(data), nbytes))
+            arrs[d.get("Name")] = np.frombuffer(data, dtype=dt).copy()
+        out.append(arrs)
+    return out
 
 
 def write_vtu(path, op, pos, radius, quat, order, density, name="stkmob"):
@@ -133,7 +165,8 @@ def write_vtu(path, op, pos, radius, quat, order, density, name="stkmob"):
     the density (3 components, the grid values as given and the poles from the
     SH expansion, `op.pole_values`), the quadrature weights (0 at the poles) and
     the triangle/quad cells of grid_with_poles_cells.  `op` is the
-    SphereSTKSHOperator of these spheres.  ASCII data arrays."""
+    SphereSTKSHOperator of these spheres.  Data arrays are inline base64
+    binary as the reference writes them (Shexp.cpp:182-195)."""
     pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
     radius = np.asarray(radius, dtype=np.float64).ravel()
     quat = np.asarray(quat, dtype=np.float64).reshape(-1, 4)

@@ -275,30 +275,6 @@ def test_inner_level_orders_from_environment(monkeypatch):
             inner(x)
 
 
-def test_far_traction_mfma_path_matches():
-    """The opt-in matrix-core FP64 traction far field (SPHSTK_FAR_MFMA=1:
-    sphere-aligned work items, dot products on v_mfma_f64_16x16x4f64) agrees
-    with the oracle like the default vector path, on mixed orders and radii
-    (partial 16-target column tiles, interval tails)."""
-    import os
-    say("far traction MFMA path")
-    pos, rad, quat, order = small_geometry(seed=41, n=12, orders=(5, 6, 8, 7, 10))
-    ref = O.Operator(pos, rad, quat, order)
-    x = np.random.default_rng(12).standard_normal(ref.dof)
-    want = ref.apply_p2p(x, None, 0.0, 0.0, 1.0)
-    old = os.environ.get("SPHSTK_FAR_MFMA")
-    os.environ["SPHSTK_FAR_MFMA"] = "1"
-    try:
-        gpu = S.SphereSTKSHOperator(pos, rad, quat, order)
-    finally:
-        if old is None:
-            del os.environ["SPHSTK_FAR_MFMA"]
-        else:
-            os.environ["SPHSTK_FAR_MFMA"] = old
-    got = gpu.applyP2POP(x, np.zeros(ref.dof), 0.0, 0.0, 1.0)
-    assert relmax(got, want) < 1e-11
-
-
 @pytest.mark.parametrize("p", [10, 12])
 def test_outer_fp64_operator_accuracy(p):
     """The FP64 multi-level outer operator of the mixed form (pairs beyond
@@ -413,10 +389,11 @@ def test_pole_values_and_vtu_output(tmp_path):
     S.system.write_vtu(f, op, pos, rad, quat, order, x)
     pieces = ET.parse(f).getroot().find("UnstructuredGrid").findall("Piece")
     assert len(pieces) == len(rad)
+    arrays = S.system.read_vtu_arrays(f)
     for s, piece in enumerate(pieces):
         p = int(order[s])
         assert int(piece.get("NumberOfPoints")) == 2 * p * p + 4 * p + 4
-        arr = {d.get("Name"): np.array(d.text.split(), float) for d in piece.iter("DataArray")}
+        arr = arrays[s]
         vals = arr["stkmob"].reshape(-1, 3)
         assert np.allclose(vals[0], poles[s, 0], rtol=0, atol=1e-14)
         assert np.allclose(vals[-1], poles[s, 1], rtol=0, atol=1e-14)
@@ -513,28 +490,20 @@ def test_self_inverse_is_exact_for_an_isolated_sphere(R, axis, ang, p):
 
 
 def test_self_inverse_preconditioned_mixed_solve():
-    """With the inner solve right-preconditioned by M^-1 the mixed solve still meets
-    the 1e-7 bar against the FP64 oracle, in fewer inner iterations."""
-    import os
+    """With the inner solve right-preconditioned by M^-1 (SPHSTK_FLAG_INNER_PRECOND)
+    the mixed solve still meets the 1e-7 bar against the FP64 oracle, in fewer
+    inner iterations."""
     say("mixed solve with the isolated-sphere preconditioner")
     pos, rad, quat, order = small_geometry(seed=41, n=8, orders=(6, 8, 7))
     F = np.random.default_rng(12).standard_normal(48)
     vr, _ = O.MobilityOperator(pos, rad, quat, order).solve(F, tol=1e-10)
     its = {}
-    old = os.environ.get("SPHSTK_INNER_PRECOND")
-    try:
-        for flag in ("0", "1"):
-            os.environ["SPHSTK_INNER_PRECOND"] = flag
-            M = S.SphereSTKMobMat(pos, rad, quat, order, precision=S.PREC_MIXED)
-            v, info = M.solveMob(F)
-            assert info["converged"] == 1 and rel(v, vr) < 1e-7
-            its[flag] = info["iters"]
-    finally:
-        if old is None:
-            os.environ.pop("SPHSTK_INNER_PRECOND", None)
-        else:
-            os.environ["SPHSTK_INNER_PRECOND"] = old
-    assert its["1"] < its["0"]
+    for flag in (False, True):
+        M = S.SphereSTKMobMat(pos, rad, quat, order, precision=S.PREC_MIXED, inner_precond=flag)
+        v, info = M.solveMob(F)
+        assert info["converged"] == 1 and rel(v, vr) < 1e-7
+        its[flag] = info["iters"]
+    assert its[True] < its[False]
 
 
 @pytest.mark.parametrize("prec,tol", [(S.PREC_FP64, 1e-10), (S.PREC_MIXED, 1e-7)])

@@ -87,3 +87,59 @@ def test_grid_with_poles_cells_cover_the_sphere():
         assert set(edges.values()) == {2}
         # Euler characteristic of a sphere: V - E + F = 2
         assert npts - len(edges) + len(typ) == 2
+
+
+class _GridOp:
+    """Stand-in for SphereSTKSHOperator's grid() and pole_values(): two spheres
+    with arbitrary (but known) grid values, no GPU."""
+
+    def __init__(self, orders):
+        self.orders = orders
+        n = [(p + 1) * (2 * p + 2) for p in orders]
+        self.index = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.int64)
+        self.length = np.array(n, dtype=np.int64)
+        tot = int(sum(n))
+        rng = np.random.default_rng(3)
+        self.coords = rng.standard_normal(3 * tot)
+        self.weights = rng.random(tot)
+        self.poles = rng.standard_normal((len(orders), 2, 3))
+
+    def grid(self):
+        return dict(weights=self.weights, coords=self.coords, index=self.index, length=self.length)
+
+    def pole_values(self, density):
+        return self.poles
+
+
+def test_vtu_is_base64_binary_and_round_trips(tmp_path):
+    """write_vtu writes every DataArray as inline base64 binary (UInt32 byte count,
+    then the values) like the reference's writer, and read_vtu_arrays returns the
+    exact values: grid values between the two pole values, weights 0 at the poles,
+    points with the poles at c +- R e_z (identity frame), the cell arrays."""
+    import xml.etree.ElementTree as ET
+    orders = [3, 5]
+    op = _GridOp(orders)
+    pos = np.array([[0.0, 1.0, 2.0], [5.0, -1.0, 0.5]])
+    rad = np.array([1.0, 0.7])
+    quat = np.tile([1.0, 0, 0, 0], (2, 1))
+    dens = np.random.default_rng(4).standard_normal(3 * int(op.length.sum()))
+    f = tmp_path / "d.vtu"
+    S.system.write_vtu(f, op, pos, rad, quat, orders, dens)
+    for d in ET.parse(f).getroot().iter("DataArray"):
+        assert d.get("format") == "binary"
+        float(len(d.text.strip()))  # one base64 token, no spaces
+        assert " " not in d.text.strip()
+    arrays = S.system.read_vtu_arrays(f)
+    assert len(arrays) == 2
+    for s, a in enumerate(arrays):
+        i0, n = int(op.index[s]), int(op.length[s])
+        vals = a["stkmob"].reshape(-1, 3)
+        assert np.array_equal(vals[0], op.poles[s, 0]) and np.array_equal(vals[-1], op.poles[s, 1])
+        assert np.array_equal(vals[1:-1], dens.reshape(-1, 3)[i0:i0 + n])
+        w = a["weights"]
+        assert w[0] == 0 and w[-1] == 0 and np.array_equal(w[1:-1], op.weights[i0:i0 + n])
+        pts = a["points"].reshape(-1, 3)
+        assert np.array_equal(pts[0], pos[s] + [0, 0, rad[s]]) and np.array_equal(pts[-1], pos[s] - [0, 0, rad[s]])
+        conn, off, typ = S.system.grid_with_poles_cells(orders[s])
+        assert a["connectivity"].dtype == np.int32 and np.array_equal(a["connectivity"], conn)
+        assert np.array_equal(a["offsets"], off) and np.array_equal(a["types"], typ)
