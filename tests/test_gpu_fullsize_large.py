@@ -1,0 +1,291 @@
+"""Parity at the full size of the three large workloads, against the oracle.
+
+The oracle cannot solve these in test time, so (as for the 1024-sphere case)
+its sphere-range apply checks the GPU on sampled target spheres with every
+sphere acting as a source, and whole-suspension oracle solves are used where
+they fit (the batched workload's small suspensions):
+
+* 4096 spheres, p=16, vf 0.1 (2.37 M grid points): the FP64 operator apply,
+  traction and single layer, to 1e-11 (the first oracle check of p=16); the
+  mixed form's density satisfies the oracle's A x = b (BIE/SphereSTKMobMat.cpp:196,
+  :222-237) on the sampled spheres to 1e-7, and its velocities equal the
+  oracle's rigid-body averages of S[rho0 + x] (:253-285) to 1e-9.
+* 256 batched suspensions (32,454 spheres, p in {6..16}): the batched FP64 and
+  mixed solves against a full oracle solve of every suspension of at most 64
+  spheres (1e-10 / 1e-7), the largest suspensions on sampled spheres.
+* 16384 spheres, p=12, vf 0.2 with chains (5.54 M grid points, 9.6% of near
+  pairs at gap < 0.1 R): the FP64 apply on the spheres of the tightest pairs
+  to 1e-11; the mixed density's oracle residual on the same spheres to 1e-7;
+  and the mixed solution's residual against the FP64 direct-sum operator over
+  all 16.6 M unknowns (the refinement certifies it against the multi-level FP64
+  operator; this measures it against the true one).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+import spheresimulator_amd as S
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print("[case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def grid_offsets(c):
+    n = (c["order"].astype(np.int64) + 1) * (2 * c["order"].astype(np.int64) + 2)
+    return np.concatenate([[0], np.cumsum(n)])
+
+
+def rows_of(off, spheres):
+    return np.concatenate([np.arange(3 * off[i], 3 * off[i + 1]) for i in spheres])
+
+
+def sampled(ref, x, spheres, cId, cSL, cTrac, cLOP=0.0):
+    """The oracle's cId x + cSL S[x] + cTrac K'[x] (+ cLOP L[x]) on the listed spheres."""
+    out = np.zeros(ref.dof)
+    for i in spheres:
+        ref.apply_p2p_spheres(x, i, i + 1, out, cId, cSL, cTrac)
+    if cLOP:
+        ref.apply_lop(x, out, cLOP)
+    return out
+
+
+def rho0(c, norms, off):
+    """rho0 = F/(4 pi R^2) + 3/(8 pi R^4) T x (R n)   (MobMat.cpp:174-192)."""
+    F = c["force"].reshape(-1, 6)
+    sph = np.repeat(np.arange(len(c["radius"])), np.diff(off))
+    R = c["radius"][sph][:, None]
+    return (F[sph, :3] / (4 * np.pi * R ** 2) +
+            3.0 / (8 * np.pi * R ** 4) * np.cross(F[sph, 3:], R * norms.reshape(-1, 3))).ravel()
+
+
+def rigid_averages(c, norms, w, off, u, spheres, mu=1.0):
+    """V = sum w u / (4 pi R^2 mu), Omega = 3 sum w (R n) x u / (8 pi R^4 mu)."""
+    out = []
+    nr = norms.reshape(-1, 3)
+    for i in spheres:
+        a, b = off[i], off[i + 1]
+        R = c["radius"][i]
+        ui = u[3 * a:3 * b].reshape(-1, 3)
+        wi = w[a:b, None]
+        V = (wi * ui).sum(0) / (4 * np.pi * R ** 2 * mu)
+        Om = 3.0 * (wi * np.cross(R * nr[a:b], ui)).sum(0) / (8 * np.pi * R ** 4 * mu)
+        out.append(np.concatenate([V, Om]))
+    return np.array(out)
+
+
+def oracle_residual(ref, c, norms, off, x, spheres):
+    """||b - A x|| / ||b|| on the listed spheres, b = -(1/2 I + K') rho0, A = 1/2 I + K' + L (oracle)."""
+    rows = rows_of(off, spheres)
+    b = sampled(ref, rho0(c, norms, off), spheres, -0.5, 0.0, -1.0)[rows]
+    r = b - sampled(ref, x, spheres, 0.5, 0.0, 1.0, 1.0)[rows]
+    return float(np.linalg.norm(r) / np.linalg.norm(b))
+
+
+# ---------------------------------------------------------------------------
+# 4096 spheres, p = 16
+
+CFG3_SAMPLE = (0, 1, 1365, 2048, 2731, 4094, 4095)
+
+
+@pytest.fixture(scope="module")
+def cfg3():
+    say("4096 spheres p=16: inputs and oracle operator")
+    c = S.configs.config3()
+    ref = O.Operator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+    coords, norms, w, off = ref.grid()
+    return dict(c=c, ref=ref, norms=norms.ravel(), w=w, off=off)
+
+
+def test_cfg3_fp64_apply_parity_p16(cfg3):
+    c, ref, off = cfg3["c"], cfg3["ref"], cfg3["off"]
+    say("4096 spheres p=16: FP64 applies on sampled spheres")
+    gpu = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+    x = np.random.default_rng(31).standard_normal(ref.dof)
+    rows = rows_of(off, CFG3_SAMPLE)
+    err = rel(gpu.apply(x)[rows], sampled(ref, x, CFG3_SAMPLE, 0.5, 0.0, 1.0, 1.0)[rows])
+    print("A x: rel err %.3e" % err)
+    assert err < 1e-11
+    err = rel(gpu.applyP2POP(x, np.zeros(ref.dof), 0.0, 1.0, 0.0)[rows], sampled(ref, x, CFG3_SAMPLE, 0.0, 1.0, 0.0)[rows])
+    print("S x: rel err %.3e" % err)
+    assert err < 1e-11
+
+
+def test_cfg3_mixed_solve_against_oracle(cfg3):
+    c, ref, off, norms, w = cfg3["c"], cfg3["ref"], cfg3["off"], cfg3["norms"], cfg3["w"]
+    say("4096 spheres p=16: mixed solve")
+    M = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], precision=S.PREC_MIXED)
+    v, info = M.solveMob(c["force"])
+    print("mixed: outer %d inner iters %d relres %.3e" % (info["outer"], info["iters"], info["relres"]))
+    assert info["converged"] == 1 and info["relres"] <= 1e-8
+    x, rho = M.get_unknown(), M.getDensitySolution()
+    assert rel(rho - x, rho0(c, norms, off)) < 1e-14
+    res = oracle_residual(ref, c, norms, off, x, CFG3_SAMPLE)
+    print("oracle residual of the mixed density on sampled spheres %.3e" % res)
+    assert res < 1e-7
+    u = sampled(ref, rho, CFG3_SAMPLE, 0.0, 1.0, 0.0)
+    want = rigid_averages(c, norms, w, off, u, CFG3_SAMPLE)
+    got = v.reshape(-1, 6)[list(CFG3_SAMPLE)]
+    eV, eW = rel(got[:, :3], want[:, :3]), rel(got[:, 3:], want[:, 3:])
+    print("mixed velocities vs the oracle's averages of S[rho]: V %.3e Omega %.3e" % (eV, eW))
+    assert eV < 1e-9 and eW < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# 256 batched suspensions
+
+@pytest.fixture(scope="module")
+def cfg4():
+    say("256 suspensions: inputs, batched FP64 and mixed solves")
+    c = S.configs.config4()
+    off = np.concatenate([[0], np.cumsum(c["counts"])]).astype(np.int64)
+    out = dict(c=c, off=off)
+    for prec in (S.PREC_FP64, S.PREC_MIXED):
+        M = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], counts=c["counts"], precision=prec)
+        v, info = M.solveMob(c["force"])
+        print("prec %d: iters %d outer %d relres %.3e converged %d" %
+              (prec, info["iters"], info["outer"], info["relres"], info["converged"]))
+        assert info["converged"] == 1
+        out[prec] = dict(v=v, x=M.get_unknown(), rho=M.getDensitySolution())
+        del M
+    return out
+
+
+SMALL_CHUNKS = 6
+
+
+def small_suspensions(c):
+    return [b for b in range(len(c["counts"])) if c["counts"][b] <= 64]
+
+
+@pytest.mark.parametrize("chunk", range(SMALL_CHUNKS))
+def test_cfg4_small_suspensions_match_oracle_solves(cfg4, chunk):
+    """Every suspension of at most 64 spheres (122 of 256), in six groups: the
+    batched FP64 solve equals the oracle's solve of that suspension to 1e-10,
+    the batched mixed solve to 1e-7."""
+    c, off = cfg4["c"], cfg4["off"]
+    small = small_suspensions(c)
+    mine = small[chunk::SMALL_CHUNKS]
+    say("256 suspensions: oracle solves of %d small suspensions (group %d)" % (len(mine), chunk))
+    worst = {S.PREC_FP64: 0.0, S.PREC_MIXED: 0.0}
+    for b in mine:
+        s = slice(off[b], off[b + 1])
+        vr, info = O.MobilityOperator(c["pos"][s], c["radius"][s], c["quat"][s], c["order"][s]).solve(
+            c["force"][6 * off[b]:6 * off[b + 1]], tol=1e-8)
+        for prec in worst:
+            e = rel(cfg4[prec]["v"][6 * off[b]:6 * off[b + 1]], vr)
+            worst[prec] = max(worst[prec], e)
+    print("worst rel err: FP64 %.3e mixed %.3e" % (worst[S.PREC_FP64], worst[S.PREC_MIXED]))
+    assert worst[S.PREC_FP64] < 1e-10
+    assert worst[S.PREC_MIXED] < 1e-7
+
+
+def test_cfg4_largest_suspensions_on_sampled_spheres(cfg4):
+    """The four largest suspensions (481-503 spheres): the FP64 and mixed
+    densities satisfy the oracle's equation on sampled spheres (1e-7) and the
+    FP64 velocities are the oracle's averages of S[rho] (1e-11)."""
+    c, off = cfg4["c"], cfg4["off"]
+    big = list(np.argsort(c["counts"])[-4:])
+    for b in big:
+        s = slice(off[b], off[b + 1])
+        n = int(c["counts"][b])
+        say("256 suspensions: suspension %d (%d spheres, p=%d) on sampled spheres" % (b, n, c["order"][off[b]]))
+        sub = dict(pos=c["pos"][s], radius=c["radius"][s], quat=c["quat"][s], order=c["order"][s],
+                   force=c["force"][6 * off[b]:6 * off[b + 1]])
+        ref = O.Operator(sub["pos"], sub["radius"], sub["quat"], sub["order"], 0.5, 0.0, 1.0, 1.0)
+        _, norms, w, goff = ref.grid()
+        norms = norms.ravel()
+        sample = (0, n // 2, n - 1)
+        g0 = 3 * grid_offsets(c)[off[b]]
+        for prec, tol_res in ((S.PREC_FP64, 1e-7), (S.PREC_MIXED, 1e-7)):
+            x = cfg4[prec]["x"][g0:g0 + ref.dof]
+            res = oracle_residual(ref, sub, norms, goff, x, sample)
+            print("  prec %d oracle residual %.3e" % (prec, res))
+            assert res < tol_res
+        rho = cfg4[S.PREC_FP64]["rho"][g0:g0 + ref.dof]
+        want = rigid_averages(sub, norms, w, goff, sampled(ref, rho, sample, 0.0, 1.0, 0.0), sample)
+        got = cfg4[S.PREC_FP64]["v"][6 * off[b]:6 * off[b + 1]].reshape(-1, 6)[list(sample)]
+        e = rel(got, want)
+        print("  FP64 velocities vs oracle averages of S[rho]: %.3e" % e)
+        assert e < 1e-11
+
+
+# ---------------------------------------------------------------------------
+# 16384 spheres, p = 12, dense with chains
+
+@pytest.fixture(scope="module")
+def cfg5():
+    say("16384 spheres p=12: inputs and oracle operator")
+    c = S.configs.config5()
+    ref = O.Operator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+    _, norms, w, off = ref.grid()
+    t, s = ref.near_pairs()
+    R = c["radius"]
+    gap = np.linalg.norm(c["pos"][t] - c["pos"][s], axis=1) - R[t] - R[s]
+    tight = []
+    for k in np.argsort(gap):
+        for i in (int(t[k]), int(s[k])):
+            if i not in tight:
+                tight.append(i)
+        if len(tight) >= 6:
+            break
+    sample = tuple(tight[:6]) + (0, len(R) - 1)
+    print("sampled spheres %s, tightest gap %.4f" % (sample, gap.min()))
+    return dict(c=c, ref=ref, norms=norms.ravel(), w=w, off=off, sample=sample)
+
+
+def test_cfg5_fp64_apply_parity_tight_pairs(cfg5):
+    c, ref, off, sample = cfg5["c"], cfg5["ref"], cfg5["off"], cfg5["sample"]
+    say("16384 spheres p=12: FP64 apply on the spheres of the tightest pairs")
+    gpu = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+    x = np.random.default_rng(51).standard_normal(ref.dof)
+    rows = rows_of(off, sample)
+    err = rel(gpu.apply(x)[rows], sampled(ref, x, sample, 0.5, 0.0, 1.0, 1.0)[rows])
+    print("A x: rel err %.3e" % err)
+    assert err < 1e-11
+
+
+@pytest.fixture(scope="module")
+def cfg5_mixed(cfg5):
+    c = cfg5["c"]
+    say("16384 spheres p=12: mixed solve")
+    M = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], precision=S.PREC_MIXED)
+    v, info = M.solveMob(c["force"])
+    print("mixed: outer %d inner iters %d relres %.3e" % (info["outer"], info["iters"], info["relres"]))
+    return dict(v=v, info=info, x=M.get_unknown(), rho=M.getDensitySolution())
+
+
+def test_cfg5_mixed_solution_against_the_fp64_operator(cfg5, cfg5_mixed):
+    """The mixed solution's residual against the FP64 direct-sum operator and
+    right-hand side, over all unknowns (two FP64 applies on the GPU)."""
+    c, off, norms = cfg5["c"], cfg5["off"], cfg5["norms"]
+    info = cfg5_mixed["info"]
+    assert info["converged"] == 1 and info["relres"] <= 1e-8
+    say("16384 spheres p=12: residual of the mixed solution against the FP64 operator")
+    gpu = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+    b = gpu.applyP2POP(rho0(c, norms, off), np.zeros(gpu.n_dof), -0.5, 0.0, -1.0)
+    r = b - gpu.apply(cfg5_mixed["x"])
+    res = float(np.linalg.norm(r) / np.linalg.norm(b))
+    print("||b64 - A64 x|| / ||b64|| = %.3e (refinement relres against the multi-level operator %.3e)" %
+          (res, info["relres"]))
+    assert res < 1e-8
+
+
+def test_cfg5_mixed_solution_on_tight_pairs(cfg5, cfg5_mixed):
+    c, ref, off, norms, w, sample = (cfg5[k] for k in ("c", "ref", "off", "norms", "w", "sample"))
+    say("16384 spheres p=12: oracle residual and velocities of the mixed solution on tight pairs")
+    res = oracle_residual(ref, c, norms, off, cfg5_mixed["x"], sample)
+    print("oracle residual on the sampled spheres %.3e" % res)
+    assert res < 1e-7
+    u = sampled(ref, cfg5_mixed["rho"], sample, 0.0, 1.0, 0.0)
+    want = rigid_averages(c, norms, w, off, u, sample)
+    got = cfg5_mixed["v"].reshape(-1, 6)[list(sample)]
+    eV, eW = rel(got[:, :3], want[:, :3]), rel(got[:, 3:], want[:, 3:])
+    print("mixed velocities vs the oracle's averages of S[rho]: V %.3e Omega %.3e" % (eV, eW))
+    assert eV < 1e-9 and eW < 1e-9
