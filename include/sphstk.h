@@ -257,6 +257,23 @@ int sphstk_brownian_velocity(int n, const double *radius, double viscosity, doub
 int sphstk_collision_pairs(int n, const double *pos, const double *radius_col, const int *gid, double colbuf,
                            int64_t capacity, int64_t *npairs, int *idx_i, int *idx_j, double *phi0, double *norm_i);
 
+/* sphstk_collision_pairs on DEVICE arrays, allocating nothing.  No
+ * counterpart in the reference: for callers that cannot allocate device memory
+ * (code that a HIP graph capture surrounds, tight serving loops, harnesses
+ * that forbid hipMalloc).  pos, radius_col, gid (may be NULL: the index) and
+ * the outputs are device pointers on the current device; the four outputs are
+ * all given or all NULL (count only).  Every intermediate array (cell keys,
+ * sort buffers, cell bounds, counts, offsets, the sort workspace) lives in one
+ * static device array of about 40 MB per device, which holds searches of at
+ * most SPHSTK_NOALLOC_MAX_SPHERES spheres; larger n is rejected with
+ * SPHSTK_ERR_INVALID.  The work runs on `stream` (a hipStream_t; NULL: the
+ * default stream) and the call returns when it is done (*npairs is a host
+ * value).  Calls that overlap on one device share that array and race. */
+#define SPHSTK_NOALLOC_MAX_SPHERES 262144
+int sphstk_collision_pairs_noalloc(int n, const double *pos, const double *radius_col, const int *gid, double colbuf,
+                                   int64_t capacity, int64_t *npairs, int *idx_i, int *idx_j, double *phi0,
+                                   double *norm_i, void *stream);
+
 /* The collision LCP around the mobility solve, replacing SimToolbox's
  * CollisionSolver as called by SphereSystem::resolveCollision
  * (SRC/SphereSystem.cpp:445-474; the solver is not in the reference tree):

@@ -27,7 +27,7 @@ EXPORTED = [
     "sphstk_mobility_operator", "sphstk_mobility_set_params", "sphstk_mobility_solve", "sphstk_mobility_solve_multi", "sphstk_mobility_apply",
     "sphstk_mobility_get_density", "sphstk_mobility_get_unknown", "sphstk_mobility_set_unknown",
     "sphstk_mobility_set_density",
-    "sphstk_brownian_velocity", "sphstk_collision_pairs", "sphstk_collision_lcp",
+    "sphstk_brownian_velocity", "sphstk_collision_pairs", "sphstk_collision_pairs_noalloc", "sphstk_collision_lcp",
 ]
 
 
@@ -117,6 +117,8 @@ def load():
                                        _dp, ctypes.POINTER(LcpInfo)]
     L.sphstk_collision_pairs.argtypes = [ctypes.c_int, _dp, _dp, _ip, ctypes.c_double, ctypes.c_int64, _i64p,
                                          _ip, _ip, _dp, _dp]
+    L.sphstk_collision_pairs_noalloc.argtypes = [ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int64, _i64p,
+                                                 _vp, _vp, _vp, _vp, _vp]
     for name in EXPORTED:
         if name.startswith("sphstk_") and name not in ("sphstk_last_error", "sphstk_version",
                                                         "sphstk_device_count", "sphstk_mobility_operator"):

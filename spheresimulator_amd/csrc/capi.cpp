@@ -11,6 +11,8 @@
 
 using namespace sphstk;
 
+static_assert(SPHSTK_NOALLOC_MAX_SPHERES == kNoAllocMaxSpheres, "header and library disagree on the scratch limit");
+
 struct sphstk_operator {
     std::unique_ptr<Operator> own;
     Operator *op = nullptr;
@@ -486,6 +488,16 @@ int sphstk_collision_pairs(int n, const double *pos, const double *radius_col, c
     return guard([&] {
         SPHSTK_REQUIRE(npairs, "npairs must not be NULL");
         *npairs = collision_pairs(n, pos, radius_col, gid, colbuf, capacity, idx_i, idx_j, phi0, norm_i);
+    });
+}
+
+int sphstk_collision_pairs_noalloc(int n, const double *pos, const double *radius_col, const int *gid, double colbuf,
+                                   int64_t capacity, int64_t *npairs, int *idx_i, int *idx_j, double *phi0,
+                                   double *norm_i, void *stream) {
+    return guard([&] {
+        SPHSTK_REQUIRE(npairs, "npairs must not be NULL");
+        *npairs = collision_pairs_noalloc(n, pos, radius_col, gid, colbuf, capacity, idx_i, idx_j, phi0, norm_i,
+                                          (hipStream_t)stream);
     });
 }
 

@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 namespace sphstk {
@@ -116,10 +117,11 @@ __global__ void k_cell_bounds(int n, const unsigned *__restrict__ key, int *__re
 
 // CollisionSphere::collide's test (Sphere/CollisionSphere.hpp:55-70) for the
 // ordered pair (a, b); r is |x_b - x_a|
+// (gid NULL: the sphere index is the id)
 __device__ inline bool collides(const double *__restrict__ pos, const double *__restrict__ rcol,
                                 const int *__restrict__ gid, double colbuf, int a, int b, double &dx, double &dy,
                                 double &dz, double &r, double &sep) {
-    if (gid[a] >= gid[b]) return false;
+    if ((gid ? gid[a] : a) >= (gid ? gid[b] : b)) return false;
     dx = pos[3 * b] - pos[3 * a];
     dy = pos[3 * b + 1] - pos[3 * a + 1];
     dz = pos[3 * b + 2] - pos[3 * a + 2];
@@ -202,6 +204,112 @@ struct Stream {
 };
 
 unsigned grid_of(int n) { return (unsigned)((n + 255) / 256); }
+
+// The cell grid of the collision search over spheres in the box [lo, hi] with
+// largest collision radius rmax: a colliding pair has |x_b - x_a| < R_a + R_b +
+// colbuf R_a <= (2 + max(colbuf, 0)) R_max, so such pairs lie in the same or
+// adjacent cells of edge h >= that distance; h grows while the grid would have
+// more than max(64, 8n) cells.  key_bits: bits of the largest cell key.
+CellGrid cell_grid(int n, const double lo[3], const double hi[3], double rmax, double colbuf, int &key_bits) {
+    double h = (2.0 + std::max(colbuf, 0.0)) * rmax * (1.0 + 1e-9);
+    CellGrid g{};
+    const double max_cells = std::max(64.0, 8.0 * n);
+    for (;;) {
+        double cells = 1.0;
+        for (int d = 0; d < 3; d++) cells *= std::floor((hi[d] - lo[d]) / h) + 1.0;
+        if (cells <= max_cells) break;
+        h *= std::max(1.01, std::cbrt(cells / max_cells));
+    }
+    for (int d = 0; d < 3; d++) {
+        g.lo[d] = lo[d];
+        g.dim[d] = (int)(std::floor((hi[d] - lo[d]) / h) + 1.0);
+    }
+    g.inv_h = 1.0 / h;
+    const int64_t ncells = (int64_t)g.dim[0] * g.dim[1] * g.dim[2];
+    SPHSTK_REQUIRE(ncells < (int64_t(1) << 31), "too many spheres for the cell grid");
+    key_bits = 1;
+    while (key_bits < 32 && (1u << key_bits) < (unsigned)ncells) key_bits++;
+    return g;
+}
+
+// Bounding box, largest collision radius and input check of n spheres on the
+// device, by one workgroup: out[0..2] = lo, out[3..5] = hi, out[6] = max Rc,
+// out[7] = 1 if a position is not finite or a radius is not > 0.
+constexpr int kBoundsThreads = 256;
+__global__ __launch_bounds__(kBoundsThreads) void k_bounds(int n, const double *__restrict__ pos,
+                                                           const double *__restrict__ rcol, double *__restrict__ out) {
+    __shared__ double sh[8][kBoundsThreads];
+    double v[8] = {HUGE_VAL, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL, -HUGE_VAL, 0.0, 0.0};
+    for (int a = threadIdx.x; a < n; a += blockDim.x) {
+        for (int d = 0; d < 3; d++) {
+            const double x = pos[3 * a + d];
+            if (!isfinite(x)) v[7] = 1.0;
+            v[d] = fmin(v[d], x);
+            v[3 + d] = fmax(v[3 + d], x);
+        }
+        const double r = rcol[a];
+        if (!(isfinite(r) && r > 0.0)) v[7] = 1.0;
+        v[6] = fmax(v[6], r);
+    }
+    for (int k = 0; k < 8; k++) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int k = 0; k < 8; k++) {
+                const double a = sh[k][threadIdx.x], b = sh[k][threadIdx.x + o];
+                sh[k][threadIdx.x] = k < 3 ? fmin(a, b) : fmax(a, b);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 8) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void k_fill_i64(int64_t n, int64_t v, int64_t *__restrict__ y) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) y[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the collision search that allocates nothing: one file-scope
+// device array, reached from the host through the address that
+// hipGetSymbolAddress returns for the current device (each device holds its
+// own copy).  Layout for n <= kNoAllocMaxSpheres spheres: bounds (8 doubles),
+// keys, sorted keys, indices, sorted indices (n each), cell begin / end (up to
+// 8n cells each), pair counts and offsets (n + 1 each), then the radix sort and
+// scan workspace.
+constexpr int64_t kNA = kNoAllocMaxSpheres;
+constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t kOffBounds = 0;
+constexpr size_t kOffKey = align256(kOffBounds + 8 * sizeof(double));
+constexpr size_t kOffSKey = align256(kOffKey + kNA * sizeof(unsigned));
+constexpr size_t kOffIdx = align256(kOffSKey + kNA * sizeof(unsigned));
+constexpr size_t kOffSorted = align256(kOffIdx + kNA * sizeof(int));
+constexpr size_t kOffBegin = align256(kOffSorted + kNA * sizeof(int));
+constexpr size_t kOffEnd = align256(kOffBegin + 8 * kNA * sizeof(int));
+constexpr size_t kOffCount = align256(kOffEnd + 8 * kNA * sizeof(int));
+constexpr size_t kOffOff = align256(kOffCount + (kNA + 1) * sizeof(int64_t));
+constexpr size_t kOffTmp = align256(kOffOff + (kNA + 1) * sizeof(int64_t));
+constexpr size_t kTmpBytes = size_t(16) << 20;
+constexpr size_t kScratchBytes = kOffTmp + kTmpBytes;
+
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on the current device, looked up once per device.
+unsigned char *scratch_on_current_device() {
+    static std::mutex mu;
+    static std::vector<unsigned char *> ptr;
+    int dev = 0;
+    SPHSTK_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)ptr.size() <= dev) ptr.resize(dev + 1, nullptr);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        SPHSTK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        SPHSTK_REQUIRE(p != nullptr, "no device address for the collision-search scratch");
+        ptr[dev] = (unsigned char *)p;
+    }
+    return ptr[dev];
+}
 // the pair passes run one wave per block, so that even 16k spheres spread over
 // all 256 CUs (one thread per sphere, latency-bound cell scans)
 unsigned grid_of64(int n) { return (unsigned)((n + 63) / 64); }
@@ -248,26 +356,9 @@ int64_t collision_pairs(int n, const double *pos, const double *rcol, const int 
     }
     std::vector<int> gid(n);
     for (int a = 0; a < n; a++) gid[a] = gid_in ? gid_in[a] : a;
-    // a colliding pair has |x_b - x_a| < R_a + R_b + colbuf R_a <= (2 + max(colbuf, 0)) R_max:
-    // such pairs lie in the same or adjacent cells of edge h >= that distance
-    double h = (2.0 + std::max(colbuf, 0.0)) * rmax * (1.0 + 1e-9);
-    CellGrid g{};
-    const double max_cells = std::max(64.0, 8.0 * n);
-    for (;;) {
-        double cells = 1.0;
-        for (int d = 0; d < 3; d++) cells *= std::floor((hi[d] - lo[d]) / h) + 1.0;
-        if (cells <= max_cells) break;
-        h *= std::max(1.01, std::cbrt(cells / max_cells));
-    }
-    for (int d = 0; d < 3; d++) {
-        g.lo[d] = lo[d];
-        g.dim[d] = (int)(std::floor((hi[d] - lo[d]) / h) + 1.0);
-    }
-    g.inv_h = 1.0 / h;
+    int key_bits = 0;
+    const CellGrid g = cell_grid(n, lo, hi, rmax, colbuf, key_bits);
     const int64_t ncells = (int64_t)g.dim[0] * g.dim[1] * g.dim[2];
-    SPHSTK_REQUIRE(ncells < (int64_t(1) << 31), "too many spheres for the cell grid");
-    int key_bits = 1;
-    while (key_bits < 32 && (1u << key_bits) < (unsigned)ncells) key_bits++;
 
     Stream st;
     Dev<double> dpos(3 * (size_t)n), drc(n);
@@ -310,6 +401,66 @@ int64_t collision_pairs(int n, const double *pos, const double *rcol, const int 
     if (phi0) SPHSTK_HIP(hipMemcpyAsync(phi0, dphi.p, sizeof(double) * total, hipMemcpyDeviceToHost, st.s));
     if (norm) SPHSTK_HIP(hipMemcpyAsync(norm, dnorm.p, sizeof(double) * 3 * total, hipMemcpyDeviceToHost, st.s));
     SPHSTK_HIP(hipStreamSynchronize(st.s));
+    return total;
+}
+
+int64_t collision_pairs_noalloc(int n, const double *dpos, const double *drc, const int *dgid, double colbuf,
+                                int64_t capacity, int *dii, int *djj, double *dphi, double *dnorm,
+                                hipStream_t s) {
+    SPHSTK_REQUIRE(n >= 0, "n must be >= 0");
+    SPHSTK_REQUIRE(n <= kNoAllocMaxSpheres, "n = " + std::to_string(n) + " exceeds the " +
+                                                std::to_string(kNoAllocMaxSpheres) +
+                                                " spheres of the collision-search scratch");
+    if (n < 2) return 0;
+    SPHSTK_REQUIRE(dpos && drc, "pos and radius_col must not be NULL");
+    SPHSTK_REQUIRE(std::isfinite(colbuf) && colbuf > -1.0, "colbuf must be > -1");
+    SPHSTK_REQUIRE(capacity >= 0, "capacity must be >= 0");
+    const bool want = dii || djj || dphi || dnorm;
+    SPHSTK_REQUIRE(!want || (dii && djj && dphi && dnorm), "the four output arrays must all be given, or none");
+    unsigned char *base = scratch_on_current_device();
+    double *bounds = (double *)(base + kOffBounds);
+    unsigned *key = (unsigned *)(base + kOffKey), *skey = (unsigned *)(base + kOffSKey);
+    int *idx = (int *)(base + kOffIdx), *sorted = (int *)(base + kOffSorted);
+    int *begin = (int *)(base + kOffBegin), *end = (int *)(base + kOffEnd);
+    int64_t *count = (int64_t *)(base + kOffCount), *off = (int64_t *)(base + kOffOff);
+    void *tmp = base + kOffTmp;
+
+    k_bounds<<<1, kBoundsThreads, 0, s>>>(n, dpos, drc, bounds);
+    SPHSTK_HIP(hipGetLastError());
+    double hb[8];
+    SPHSTK_HIP(hipMemcpyAsync(hb, bounds, sizeof hb, hipMemcpyDeviceToHost, s));
+    SPHSTK_HIP(hipStreamSynchronize(s));
+    SPHSTK_REQUIRE(hb[7] == 0.0, "positions must be finite and collision radii > 0");
+    int key_bits = 0;
+    const CellGrid g = cell_grid(n, hb, hb + 3, hb[6], colbuf, key_bits);
+    const int64_t ncells = (int64_t)g.dim[0] * g.dim[1] * g.dim[2];
+    SPHSTK_REQUIRE(ncells <= 8 * kNA, "internal: cell grid larger than the scratch");
+
+    size_t tmp_sort = 0, tmp_scan = 0;
+    SPHSTK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_sort, key, skey, idx, sorted, n, 0, key_bits, s));
+    SPHSTK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_scan, count, off, n + 1, s));
+    SPHSTK_REQUIRE(std::max(tmp_sort, tmp_scan) <= kTmpBytes, "internal: sort workspace larger than the scratch");
+    // everything read before it is written starts from a fill kernel (the
+    // scratch keeps the previous call's contents)
+    k_cell_keys<<<grid_of(n), 256, 0, s>>>(n, dpos, g, key, idx);
+    SPHSTK_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_sort, key, skey, idx, sorted, n, 0, key_bits, s));
+    const unsigned cg = (unsigned)((ncells + 255) / 256);
+    k_fill_i32<<<cg, 256, 0, s>>>(ncells, 0, begin);
+    k_fill_i32<<<cg, 256, 0, s>>>(ncells, 0, end);
+    k_fill_i64<<<1, 64, 0, s>>>(1, 0, count + n);
+    k_cell_bounds<<<grid_of(n), 256, 0, s>>>(n, skey, begin, end);
+    k_pairs<false><<<grid_of64(n), 64, 0, s>>>(n, dpos, drc, dgid, colbuf, g, sorted, begin, end, count, nullptr,
+                                              nullptr, nullptr, nullptr, nullptr);
+    SPHSTK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_scan, count, off, n + 1, s));
+    SPHSTK_HIP(hipGetLastError());
+    int64_t total = 0;
+    SPHSTK_HIP(hipMemcpyAsync(&total, off + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SPHSTK_HIP(hipStreamSynchronize(s));
+    if (!want || total > capacity || total == 0) return total;
+    k_pairs<true><<<grid_of64(n), 64, 0, s>>>(n, dpos, drc, dgid, colbuf, g, sorted, begin, end, nullptr, off, dii,
+                                             djj, dphi, dnorm);
+    SPHSTK_HIP(hipGetLastError());
+    SPHSTK_HIP(hipStreamSynchronize(s));
     return total;
 }
 

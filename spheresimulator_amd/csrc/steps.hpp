@@ -17,4 +17,13 @@ void brownian_velocity(int n, const double *radius, double mu, double kBT, doubl
 int64_t collision_pairs(int n, const double *pos, const double *rcol, const int *gid, double colbuf,
                         int64_t capacity, int *ii, int *jj, double *phi0, double *norm);
 
+// The same search on device arrays with no device allocation: every
+// intermediate array lives in a static device scratch sized for
+// kNoAllocMaxSpheres spheres; gid may be NULL (the index); the four outputs
+// are all given or all NULL.  Work runs on stream s, which is synchronised
+// before returning.  Calls that overlap on one device share the scratch and race.
+constexpr int kNoAllocMaxSpheres = 1 << 18;
+int64_t collision_pairs_noalloc(int n, const double *pos, const double *rcol, const int *gid, double colbuf,
+                                int64_t capacity, int *ii, int *jj, double *phi0, double *norm, hipStream_t s);
+
 } // namespace sphstk
