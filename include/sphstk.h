@@ -55,6 +55,7 @@ extern "C" {
 #define SPHSTK_FLAG_GRAPH 8       /* replay the FP32 inner operator's apply from a captured HIP graph */
 #define SPHSTK_FLAG_INNER_PRECOND 16 /* mixed form: right-precondition the inner solve by the isolated-sphere
                                         inverse (sphstk_operator_apply_self_inverse) */
+#define SPHSTK_FLAG_NO_CELLS 32   /* FP32 inner operator: no source-cell aggregation on its two coarsest levels */
 
 typedef struct sphstk_operator sphstk_operator;
 typedef struct sphstk_mobility sphstk_mobility;

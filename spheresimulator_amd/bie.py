@@ -22,6 +22,7 @@ FLAG_FP32_DIRECT = 2
 FLAG_AUTOTUNE = 4
 FLAG_GRAPH = 8
 FLAG_INNER_PRECOND = 16
+FLAG_NO_CELLS = 32
 
 
 def _d(a):
@@ -60,7 +61,7 @@ class SphereSTKSHOperator:
     """Y := beta Y + alpha (cId I + cSL S + cTrac K' + cLOP L) X on the GPU."""
 
     def __init__(self, pos, radius, quat, order, cId=0.0, cSL=0.0, cTrac=0.0, cLOP=0.0, counts=None, timing=False,
-                 fp32_direct=False, autotune=False, graph=False, _handle=None, _owner=None):
+                 fp32_direct=False, autotune=False, graph=False, cells=True, _handle=None, _owner=None):
         L = load()
         self._owner = _owner
         if _handle is not None:
@@ -76,7 +77,8 @@ class SphereSTKSHOperator:
                                                    (FLAG_TIMING if timing else 0) |
                                                    (FLAG_FP32_DIRECT if fp32_direct else 0) |
                                                    (FLAG_AUTOTUNE if autotune else 0) |
-                                                   (FLAG_GRAPH if graph else 0), ctypes.byref(h)))
+                                                   (FLAG_GRAPH if graph else 0) |
+                                                   (0 if cells else FLAG_NO_CELLS), ctypes.byref(h)))
             self._h = h
             self._own = True
         ns, npt, nd, nn = (ctypes.c_int64() for _ in range(4))
@@ -207,7 +209,7 @@ class SphereSTKMobMat:
     """Force/torque (6N) -> velocity/angular velocity (6N) through A = 0.5 I + K' + L."""
 
     def __init__(self, pos, radius, quat, order, viscosity=1.0, counts=None, precision=PREC_FP64, params=None,
-                 timing=False, fp32_direct=False, autotune=False, graph=False, inner_precond=False):
+                 timing=False, fp32_direct=False, autotune=False, graph=False, inner_precond=False, cells=True):
         L = load()
         s = _Spheres(pos, radius, quat, order, counts)
         self.n_spheres = s.n
@@ -221,7 +223,8 @@ class SphereSTKMobMat:
                                                (FLAG_FP32_DIRECT if fp32_direct else 0) |
                                                (FLAG_AUTOTUNE if autotune else 0) |
                                                (FLAG_GRAPH if graph else 0) |
-                                               (FLAG_INNER_PRECOND if inner_precond else 0), ctypes.byref(h)))
+                                               (FLAG_INNER_PRECOND if inner_precond else 0) |
+                                               (0 if cells else FLAG_NO_CELLS), ctypes.byref(h)))
         self._h = h
         self.n_dof = self.op.n_dof
 

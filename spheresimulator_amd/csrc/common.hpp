@@ -89,4 +89,17 @@ struct Resample {
 const Resample &resample_tables(int p, int pc);
 void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w);
 
+// Equivalent-source template of the far-field cell aggregation (equiv.cpp):
+// K equivalent points and Kc check points on the unit sphere (3 each), to be
+// placed at radii re*rho and rc*rho around a cell of radius rho, and
+// P (3K x 3Kc, row-major) with P G = I, G the Stokeslet matrix from the
+// equivalent to the check points of the unit cell: a cell's point forces are
+// represented outside its check sphere by rho P u_check.
+struct EquivTables {
+    int K = 0, Kc = 0;
+    double re = 0.0, rc = 0.0;
+    std::vector<double> eq, chk, P;
+};
+const EquivTables &equiv_tables();
+
 } // namespace sphstk

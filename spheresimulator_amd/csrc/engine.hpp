@@ -90,9 +90,12 @@ inline void run_append(Runs &r, int s) {
 }
 Runs runs_of(const std::vector<int> &lst, int s0, int s1, bool include);
 
+// src_susp_off: first source "sphere" (entry of soff) of each suspension when
+// the source point set is not the target's sphere set (NULL: susp_sph_off).
 void build_far_lists(FarLists &L, int nsusp, const std::vector<int64_t> &susp_sph_off,
                      const std::vector<int64_t> &toff, const std::vector<int64_t> &soff, const std::vector<Runs> &runs,
-                     int tpl, int split, int64_t want_items, const double *sph_pos = nullptr);
+                     int tpl, int split, int64_t want_items, const double *sph_pos = nullptr,
+                     const std::vector<int64_t> *src_susp_off = nullptr);
 
 // One level of the multi-level FP32 far field: per-sphere grids of a lower
 // order, their geometry, resampling tables and work lists.
@@ -109,6 +112,30 @@ struct FarLevel {
     FarLists lists;
 };
 
+// Source-cell aggregation of the two coarsest FP32 levels (the treecode part
+// of the far field): cells of kCellSize Morton-consecutive spheres, each
+// represented beyond kCellTheta cell radii by the K equivalent point forces of
+// equivalent_tables(); a (target sphere, cell) pair whose members all lie in
+// levels 2-3 of the target interacts through them on the target's grid of the
+// finer of its members' levels.
+constexpr int kCellSize = 16;
+constexpr double kCellTheta = 4.0;
+struct CellAgg {
+    int ncell = 0;
+    double pairs = 0.0;               // pair evaluations of the two cell lists (per apply)
+    double check_pairs = 0.0;         // pair evaluations of the check velocities (per apply)
+    std::vector<int64_t> susp_cell_off; // first cell of each suspension
+    DevBuf<float> chk32;              // check points as FP32 target records (zero normals)
+    DevBuf<float> eq32;               // equivalent points as FP32 source records (forces per apply)
+    DevBuf<double> chk_u;             // check velocities, 3 Kc per cell
+    DevBuf<double> P;                 // template fit, column-major (3K x 3Kc)
+    DevBuf<double> rho;               // cell radius
+    DevBuf<int> susp;                 // cell suspension
+    DevBuf<Chunk> chk_chunks;
+    FarLists chk_lists;               // check points of a cell x the fine grids of its members
+    FarLists lists[2];                // level-2 / level-3 target grids x equivalent points
+};
+
 // A multi-level far field: fine-grid sums for the pairs closer than the first
 // level's separation (mid), then one coarser grid per separation shell.
 constexpr int kNumLevels = 4;
@@ -119,6 +146,8 @@ struct LevelSet {
     int tpl = 1;             // targets per lane of its work lists
     std::vector<std::unique_ptr<FarLevel>> levels;
     FarLists mid;
+    bool cells = false;               // aggregate sources over cells on levels 2-3
+    std::unique_ptr<CellAgg> agg;
     bool ready = false;
 };
 
@@ -228,6 +257,7 @@ class Operator {
                     hipEvent_t ev_a, hipEvent_t ev_b);
 
     StreamOwner stream_owner_; // declared before every buffer, so destroyed after them
+    StreamOwner aux_owner_;    // second stream: self/near terms beside the far field
     int nsph_, nsusp_;
     int64_t npts_;
     static constexpr int kMultiCols = 8; // right-hand sides per shared far-field launch
@@ -237,7 +267,8 @@ class Operator {
     double c_[4];
     int flags_;
     int max_order_ = 0;
-    hipStream_t stream_ = nullptr;
+    hipStream_t stream_ = nullptr, aux_ = nullptr;
+    hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
     int device_ = 0;
 
     std::vector<double> pos_, rad_, quat_;
@@ -305,6 +336,9 @@ class Mobility {
   public:
     Mobility(int nsusp, const int *counts, const double *pos, const double *radius, const double *quat,
              const int *order, double viscosity, const SolverParams &prm, int precision, int flags, int ncopy = 1);
+    ~Mobility();
+    Mobility(const Mobility &) = delete;
+    Mobility &operator=(const Mobility &) = delete;
     Operator &op() { return *A_; }
     int precision() const { return prec_; }
     const SolverParams &params() const { return prm_; }
@@ -340,6 +374,13 @@ class Mobility {
     bool verbose_ = false; // SPHSTK_VERBOSE=1: residual history on stderr (Belos Verbosity 63)
     DevBuf<double> rho_, b_, x_, r_, d_, V_, h_, scal_, coefv_, force_, vel_, gmres_inner_rhs_, mdot_;
     DevBuf<int> act_;
+    // device-side GMRES least-squares state (per suspension) and the pinned
+    // count of active suspensions read one iteration late behind an event
+    DevBuf<double> d_H_, d_cs_, d_sn_, d_g_, d_bn_, d_beta_, d_relres_;
+    DevBuf<int> d_cyc_, d_kdim_, d_impok_, d_nact_;
+    int *nact_host_ = nullptr;
+    int last_kmax_ = 0, last_m_ = 0, last_prec_ = -1; // the last GMRES cycle's basis size, restart and operator
+    hipEvent_t nact_ev_[2] = {nullptr, nullptr};
 };
 
 } // namespace sphstk

@@ -394,6 +394,49 @@ void launch_far(int kind, int prec, int tpl, const void *src, const double *tcoo
 }
 
 // ---------------------------------------------------------------------------
+// Equivalent point forces of the far-field cells: g_c = rho_c P u_c for every
+// cell c (P: 3K x 3Kc, column-major; u_c: the cell's check velocities), written
+// as the force words (3..5) of the cell's FP32 source records, whose positions
+// were set at set-up.  kEqCells cells per workgroup share each load of P; their
+// check velocities are staged in LDS; thread i owns rows i, i + 256, ....
+constexpr int kEqCells = 4;
+constexpr int kEqThreads = 256;
+__global__ __launch_bounds__(kEqThreads) void k_equiv_forces(int ncell, int K, int Kc, const double *__restrict__ P,
+                                                             const double *__restrict__ u,
+                                                             const double *__restrict__ rho,
+                                                             const int *__restrict__ susp,
+                                                             const int *__restrict__ active, float *__restrict__ eq) {
+    extern __shared__ double su[]; // kEqCells x 3Kc
+    const int c0 = blockIdx.x * kEqCells;
+    const int nc = min(kEqCells, ncell - c0);
+    const int m = 3 * Kc, n = 3 * K;
+    for (int k = 0; k < nc; k++)
+        for (int j = threadIdx.x; j < m; j += blockDim.x) su[k * m + j] = u[(int64_t)(c0 + k) * m + j];
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double acc[kEqCells] = {};
+        for (int j = 0; j < m; j++) {
+            const double pij = P[(int64_t)j * n + i];
+#pragma unroll
+            for (int k = 0; k < kEqCells; k++) acc[k] = fmad(pij, su[k * m + j], acc[k]);
+        }
+        for (int k = 0; k < nc; k++) {
+            const int c = c0 + k;
+            if (active && !active[susp[c]]) continue;
+            eq[8 * ((int64_t)c * K + i / 3) + 3 + i % 3] = (float)(rho[c] * acc[k]);
+        }
+    }
+}
+
+void launch_equiv_forces(int ncell, int K, int Kc, const double *P, const double *u, const double *rho,
+                         const int *susp, const int *active, float *eq, hipStream_t s) {
+    if (ncell <= 0) return;
+    const size_t lds = sizeof(double) * kEqCells * 3 * Kc;
+    k_equiv_forces<<<dim3((ncell + kEqCells - 1) / kEqCells), dim3(kEqThreads), lds, s>>>(ncell, K, Kc, P, u, rho,
+                                                                                          susp, active, eq);
+}
+
+// ---------------------------------------------------------------------------
 // Per-sphere matrices applied to body-frame values, one tile of up to kTileSph
 // spheres of the same order per workgroup so that each matrix element is read
 // once per tile (the matrices are up to 12.7 MB at p = 16):
@@ -888,15 +931,27 @@ __global__ __launch_bounds__(256) void k_sheval_gather(DevGeom g, const int64_t 
     }
 }
 
+void launch_sheval_items(const DevGeom &g, int mode, const double *rec, const EvalItem *items, int nitems,
+                         double *scratch, double cSL, double cTrac, const int *active, hipStream_t s) {
+    if (!g.nsph || nitems <= 0) return;
+    if (mode == 0)
+        k_sheval<0><<<dim3(nitems), dim3(64), 0, s>>>(g, rec, items, scratch, nullptr, cSL, cTrac, active);
+    else
+        k_sheval<1><<<dim3(nitems), dim3(64), 0, s>>>(g, rec, items, scratch, nullptr, cSL, cTrac, active);
+}
+
+void launch_sheval_gather(const DevGeom &g, int nitems, const int64_t *trg_slot_off, const int *trg_nsrc,
+                          const double *scratch, const double *in, double *out, double cId, const int *active,
+                          hipStream_t s) {
+    if (!g.nsph || nitems <= 0) return;
+    k_sheval_gather<<<dim3(g.nsph), dim3(256), 0, s>>>(g, trg_slot_off, trg_nsrc, scratch, in, out, cId, active);
+}
+
 void launch_sheval(const DevGeom &g, int mode, const double *rec, const EvalItem *items, int nitems,
                    const int64_t *trg_slot_off, const int *trg_nsrc, double *scratch, const double *in, double *out,
                    double cSL, double cTrac, double cId, const int *active, hipStream_t s) {
-    if (!g.nsph || nitems <= 0) return;
-    if (mode == 0)
-        k_sheval<0><<<dim3(nitems), dim3(64), 0, s>>>(g, rec, items, scratch, out, cSL, cTrac, active);
-    else
-        k_sheval<1><<<dim3(nitems), dim3(64), 0, s>>>(g, rec, items, scratch, out, cSL, cTrac, active);
-    k_sheval_gather<<<dim3(g.nsph), dim3(256), 0, s>>>(g, trg_slot_off, trg_nsrc, scratch, in, out, cId, active);
+    launch_sheval_items(g, mode, rec, items, nitems, scratch, cSL, cTrac, active, s);
+    launch_sheval_gather(g, nitems, trg_slot_off, trg_nsrc, scratch, in, out, cId, active, s);
 }
 
 void launch_synthesis(const DevGeom &g, const double *rec, const EvalItem *self_items, int nitems, double *out,
@@ -1064,6 +1119,151 @@ __global__ void k_seg_multi_reduce(const double *__restrict__ partial, int nchun
 }
 
 // out[i] += sum_k part[k*stride + i], k = 0..nslot-1, over the DOF chunks of active segments
+// ---------------------------------------------------------------------------
+// GMRES least-squares bookkeeping on the device (Belos GMRES, one Krylov space
+// per suspension): with it a restart cycle needs no host synchronisation.
+//
+// k_gmres_init: start of a cycle: g_s = beta_s e_1, active = the cycle's set.
+__global__ void k_gmres_init(int B, int m, const double *__restrict__ beta, const int *__restrict__ cyc,
+                             double *__restrict__ g, int *__restrict__ cur, int *__restrict__ kdim,
+                             int *__restrict__ impok) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= B) return;
+    g[(size_t)s * (m + 1)] = beta[s];
+    cur[s] = cyc[s];
+    kdim[s] = 0;
+    impok[s] = 0;
+}
+
+// k_gmres_givens: column j of every active suspension's Hessenberg matrix from
+// the orthogonalisation products hh (two passes, then |w|^2; hsign -1 when the
+// passes stored -h), the previous Givens rotations applied to it, the new
+// rotation, the rotated right-hand side g and the implicit residual
+// |g_{j+1}| / |b|; a suspension whose residual reaches tol (or whose Krylov
+// space is exhausted) leaves the active set.  nact = suspensions still active.
+// One workgroup loops over the suspensions (no atomics).
+__global__ __launch_bounds__(256) void k_gmres_givens(int B, int j, int m, const double *__restrict__ hh,
+                                                      double hsign, double *__restrict__ H, double *__restrict__ cs,
+                                                      double *__restrict__ sn, double *__restrict__ g,
+                                                      const double *__restrict__ bn, double tol,
+                                                      int *__restrict__ cur, int *__restrict__ kdim,
+                                                      int *__restrict__ impok, double *__restrict__ relres,
+                                                      int *__restrict__ nact) {
+    __shared__ int sh[256];
+    int cnt = 0;
+    for (int s = threadIdx.x; s < B; s += blockDim.x) {
+        if (!cur[s]) continue;
+        double *Hs = H + (size_t)s * (m + 1) * m;
+        double *c = cs + (size_t)s * m, *sv = sn + (size_t)s * m, *gs = g + (size_t)s * (m + 1);
+        for (int i = 0; i <= j; i++) Hs[i * m + j] = hsign * (hh[(size_t)i * B + s] + hh[(size_t)((m + 1) + i) * B + s]);
+        const double hn = sqrt(fmax(hh[(size_t)(2 * (m + 1)) * B + s], 0.0));
+        Hs[(j + 1) * m + j] = hn;
+        for (int i = 0; i < j; i++) {
+            const double t0 = c[i] * Hs[i * m + j] + sv[i] * Hs[(i + 1) * m + j];
+            const double t1 = -sv[i] * Hs[i * m + j] + c[i] * Hs[(i + 1) * m + j];
+            Hs[i * m + j] = t0;
+            Hs[(i + 1) * m + j] = t1;
+        }
+        const double a = Hs[j * m + j], bb = Hs[(j + 1) * m + j];
+        const double den = sqrt(a * a + bb * bb);
+        c[j] = den > 0 ? a / den : 1.0;
+        sv[j] = den > 0 ? bb / den : 0.0;
+        Hs[j * m + j] = den;
+        Hs[(j + 1) * m + j] = 0.0;
+        gs[j + 1] = -sv[j] * gs[j];
+        gs[j] = c[j] * gs[j];
+        kdim[s] = j + 1;
+        const double rr = fabs(gs[j + 1]) / bn[s];
+        relres[s] = rr;
+        if (rr <= tol) impok[s] = 1;
+        if (rr <= tol || hn == 0.0)
+            cur[s] = 0;
+        else
+            cnt++;
+    }
+    sh[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *nact = sh[0];
+}
+
+// k_gmres_backsub: y_s = R_s^-1 g_s (back substitution of the rotated
+// Hessenberg system, kdim_s columns) as coef[i B + s], zero beyond kdim_s and
+// for suspensions outside the cycle.
+__global__ void k_gmres_backsub(int B, int m, const double *__restrict__ H, const double *__restrict__ g,
+                                const int *__restrict__ kdim, const int *__restrict__ cyc, double *__restrict__ coef) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= B) return;
+    for (int i = 0; i < m; i++) coef[(size_t)i * B + s] = 0.0;
+    if (!cyc[s]) return;
+    const double *Hs = H + (size_t)s * (m + 1) * m;
+    const double *gs = g + (size_t)s * (m + 1);
+    const int k = kdim[s];
+    for (int i = k - 1; i >= 0; i--) {
+        double acc = gs[i];
+        for (int q = i + 1; q < k; q++) acc -= Hs[i * m + q] * coef[(size_t)q * B + s];
+        coef[(size_t)i * B + s] = Hs[i * m + i] != 0.0 ? acc / Hs[i * m + i] : 0.0;
+    }
+}
+
+// k_gmres_project: y_s = argmin |c_s - Hbar_s y| through the QR of the last
+// cycle's Hessenberg matrix (its Givens rotations applied to c_s = V^T r, then
+// back substitution with R), as coef[i B + s]; zero for suspensions not in
+// act or with an empty space.  With A V_k = V_{k+1} Hbar this is the
+// minimum-residual correction of r within span(V_k).
+__global__ void k_gmres_project(int B, int m, const double *__restrict__ c, const double *__restrict__ H,
+                                const double *__restrict__ cs, const double *__restrict__ sn,
+                                const int *__restrict__ kdim, const int *__restrict__ act, double *__restrict__ coef) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= B) return;
+    for (int i = 0; i < m; i++) coef[(size_t)i * B + s] = 0.0;
+    const int k = kdim[s];
+    if (!act[s] || k <= 0) return;
+    const double *Hs = H + (size_t)s * (m + 1) * m;
+    const double *cv = cs + (size_t)s * m, *sv = sn + (size_t)s * m;
+    // rotated c, kept in coef's column for s (k + 1 <= m + 1 entries: the last in a register)
+    double last = c[(size_t)k * B + s];
+    for (int i = 0; i <= k - 1; i++) coef[(size_t)i * B + s] = c[(size_t)i * B + s];
+    for (int i = 0; i < k; i++) {
+        const double ti = coef[(size_t)i * B + s];
+        const double tn = (i + 1 < k) ? coef[(size_t)(i + 1) * B + s] : last;
+        coef[(size_t)i * B + s] = cv[i] * ti + sv[i] * tn;
+        const double t1 = -sv[i] * ti + cv[i] * tn;
+        if (i + 1 < k)
+            coef[(size_t)(i + 1) * B + s] = t1;
+        else
+            last = t1;
+    }
+    for (int i = k - 1; i >= 0; i--) {
+        double acc = coef[(size_t)i * B + s];
+        for (int q = i + 1; q < k; q++) acc -= Hs[i * m + q] * coef[(size_t)q * B + s];
+        coef[(size_t)i * B + s] = Hs[i * m + i] != 0.0 ? acc / Hs[i * m + i] : 0.0;
+    }
+}
+
+void launch_gmres_project(int B, int m, const double *c, const double *H, const double *cs, const double *sn,
+                          const int *kdim, const int *act, double *coef, hipStream_t s) {
+    if (B > 0) k_gmres_project<<<dim3((B + 255) / 256), dim3(256), 0, s>>>(B, m, c, H, cs, sn, kdim, act, coef);
+}
+
+void launch_gmres_init(int B, int m, const double *beta, const int *cyc, double *g, int *cur, int *kdim, int *impok,
+                       hipStream_t s) {
+    if (B > 0) k_gmres_init<<<dim3((B + 255) / 256), dim3(256), 0, s>>>(B, m, beta, cyc, g, cur, kdim, impok);
+}
+void launch_gmres_givens(int B, int j, int m, const double *hh, double hsign, double *H, double *cs, double *sn,
+                         double *g, const double *bn, double tol, int *cur, int *kdim, int *impok, double *relres,
+                         int *nact, hipStream_t s) {
+    k_gmres_givens<<<dim3(1), dim3(256), 0, s>>>(B, j, m, hh, hsign, H, cs, sn, g, bn, tol, cur, kdim, impok, relres,
+                                                 nact);
+}
+void launch_gmres_backsub(int B, int m, const double *H, const double *g, const int *kdim, const int *cyc,
+                          double *coef, hipStream_t s) {
+    if (B > 0) k_gmres_backsub<<<dim3((B + 255) / 256), dim3(256), 0, s>>>(B, m, H, g, kdim, cyc, coef);
+}
+
 __global__ __launch_bounds__(256) void k_slot_reduce(double *__restrict__ out, const double *__restrict__ part,
                                                      int nslot, int64_t stride, const Chunk *__restrict__ ch,
                                                      const int *__restrict__ active) {

@@ -65,6 +65,12 @@ void launch_far(int kind, int prec, int tpl, const void *src, const double *tcoo
                 const FarInterval *iv, const int *active, double *out, double coef, int accumulate, hipStream_t s,
                 bool levels = false);
 
+// Equivalent point forces of the far-field cells: the force words of eq
+// (FP32 source records, K per cell) = rho_c P u_c (P 3K x 3Kc column-major, u
+// 3Kc check velocities per cell); cells of inactive suspensions are skipped.
+void launch_equiv_forces(int ncell, int K, int Kc, const double *P, const double *u, const double *rho,
+                         const int *susp, const int *active, float *eq, hipStream_t s);
+
 // out_S (+)= R_S M_p R_S^-1 in_S between two point sets with per-sphere offsets
 void launch_resample(const DevGeom &g, const ResampleTabs &tab, const int64_t *in_off, const int64_t *out_off,
                      const double *in, double *out, int accumulate, const int *active, hipStream_t s,
@@ -90,6 +96,13 @@ struct EvalItem {
 void launch_sheval(const DevGeom &g, int mode, const double *rec, const EvalItem *items, int nitems,
                    const int64_t *trg_slot_off, const int *trg_nsrc, double *scratch, const double *in, double *out,
                    double cSL, double cTrac, double cId, const int *active, hipStream_t s);
+// the two halves of launch_sheval: the per-item evaluations into scratch, and
+// the per-target gather (out += sum of the target's items + cId in)
+void launch_sheval_items(const DevGeom &g, int mode, const double *rec, const EvalItem *items, int nitems,
+                         double *scratch, double cSL, double cTrac, const int *active, hipStream_t s);
+void launch_sheval_gather(const DevGeom &g, int nitems, const int64_t *trg_slot_off, const int *trg_nsrc,
+                          const double *scratch, const double *in, double *out, double cId, const int *active,
+                          hipStream_t s);
 void launch_identity(const DevGeom &g, const double *in, double *out, double cId, const int *active,
                      hipStream_t s);
 void launch_lop(const DevGeom &g, const double *in, double *out, double cLOP, const int *active, hipStream_t s);
@@ -118,6 +131,24 @@ void launch_seg_multi_reduce(const double *partial, int nchunks, const int *off,
                              const int *active, hipStream_t s);
 void launch_seg_multi_axpy(double *x, const double *V, int64_t ldv, const double *coef, int k, int nseg,
                            const Chunk *chunks, int nchunks, const int *active, hipStream_t s);
+// GMRES least-squares bookkeeping on the device, one Krylov space per
+// suspension (arrays: H B x (m+1) x m, cs/sn B x m, g B x (m+1), per-suspension
+// ints and doubles of length B): cycle start (g = beta e_1, cur = cyc), the
+// Givens update of column j from the orthogonalisation products hh (active set
+// cur shrinks as suspensions converge; *nact = those still active), and the
+// back substitution into coef[i B + s].
+void launch_gmres_init(int B, int m, const double *beta, const int *cyc, double *g, int *cur, int *kdim, int *impok,
+                       hipStream_t s);
+void launch_gmres_givens(int B, int j, int m, const double *hh, double hsign, double *H, double *cs, double *sn,
+                         double *g, const double *bn, double tol, int *cur, int *kdim, int *impok, double *relres,
+                         int *nact, hipStream_t s);
+void launch_gmres_backsub(int B, int m, const double *H, const double *g, const int *kdim, const int *cyc,
+                          double *coef, hipStream_t s);
+// the minimum-residual coefficients of r within the last cycle's Krylov space
+// span(V_k) from c = V_{k+1}^T r (c[i B + s], i <= kdim_s) and that cycle's
+// rotated Hessenberg matrix and rotations, as coef[i B + s]
+void launch_gmres_project(int B, int m, const double *c, const double *H, const double *cs, const double *sn,
+                          const int *kdim, const int *act, double *coef, hipStream_t s);
 // out += sum of nslot partial-sum slots (far-field source slices)
 void launch_slot_reduce(double *out, const double *part, int nslot, int64_t stride, const Chunk *chunks,
                         int nchunks, const int *active, hipStream_t s);

@@ -182,6 +182,57 @@ __global__ void k_pairs(int n, const double *__restrict__ pos, const double *__r
     }
 }
 
+// Near pairs of the boundary-integral operator: (t, s) of one suspension with
+// 0 < |c_s - c_t|^2 < (1.5 R_s + 1.5 R_t)^2 (MPI/NearInteraction.txx:430-441
+// with radiusNear = 1.5 R, BIE/SphereSTKSHOperator.cpp:547).  Products and
+// sums unfused, in the order of the host's scalar code.  COUNT pass: count[t]
+// and the smallest s of the suspension closer than R_s + R_t (overlapping,
+// -1 if none) in overlap[t]; FILL pass: the sources of t at off[t], ascending.
+template <bool FILL>
+__global__ void k_near(int n, const double *__restrict__ pos, const double *__restrict__ rad,
+                       const int *__restrict__ susp, CellGrid g, const int *__restrict__ sorted,
+                       const int *__restrict__ begin, const int *__restrict__ end, int64_t *__restrict__ count,
+                       int *__restrict__ overlap, const int64_t *__restrict__ off, int *__restrict__ src) {
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double xt = pos[3 * t], yt = pos[3 * t + 1], zt = pos[3 * t + 2], rt = rad[t];
+    const int bt = susp[t];
+    const int cx = cell_coord(xt, g.lo[0], g.inv_h, g.dim[0]);
+    const int cy = cell_coord(yt, g.lo[1], g.inv_h, g.dim[1]);
+    const int cz = cell_coord(zt, g.lo[2], g.inv_h, g.dim[2]);
+    int m = 0, ov = -1;
+    const int64_t o = FILL ? off[t] : 0;
+    for (int x = max(cx - 1, 0); x <= min(cx + 1, g.dim[0] - 1); x++)
+        for (int y = max(cy - 1, 0); y <= min(cy + 1, g.dim[1] - 1); y++)
+            for (int z = max(cz - 1, 0); z <= min(cz + 1, g.dim[2] - 1); z++) {
+                const int c = (x * g.dim[1] + y) * g.dim[2] + z;
+                for (int k = begin[c]; k < end[c]; k++) {
+                    const int s = sorted[k];
+                    if (s == t || susp[s] != bt) continue;
+                    const double dx = pos[3 * s] - xt, dy = pos[3 * s + 1] - yt, dz = pos[3 * s + 2] - zt;
+                    const double d2 = dx * dx + dy * dy + dz * dz;
+                    const double rs = rad[s] + rt;
+                    if (d2 < rs * rs && (ov < 0 || s < ov)) ov = s;
+                    const double rr = 1.5 * rad[s] + 1.5 * rt;
+                    if (!(d2 > 0 && d2 < rr * rr)) continue;
+                    if (FILL) {
+                        int q = m;
+                        while (q > 0 && src[o + q - 1] > s) {
+                            src[o + q] = src[o + q - 1];
+                            q--;
+                        }
+                        src[o + q] = s;
+                    }
+                    m++;
+                }
+            }
+    if (!FILL) {
+        count[t] = m;
+        overlap[t] = ov;
+    }
+}
+
 template <typename T>
 struct Dev {
     T *p = nullptr;
@@ -402,6 +453,61 @@ int64_t collision_pairs(int n, const double *pos, const double *rcol, const int 
     if (norm) SPHSTK_HIP(hipMemcpyAsync(norm, dnorm.p, sizeof(double) * 3 * total, hipMemcpyDeviceToHost, st.s));
     SPHSTK_HIP(hipStreamSynchronize(st.s));
     return total;
+}
+
+void near_pairs(int n, const double *pos, const double *rad, const double *dpos, const double *drad, const int *dsusp,
+                hipStream_t s, std::vector<int64_t> &near_off, std::vector<int> &near_src) {
+    near_off.assign(n + 1, 0);
+    near_src.clear();
+    if (n < 2) return;
+    double lo[3] = {pos[0], pos[1], pos[2]}, hi[3] = {pos[0], pos[1], pos[2]}, rmax = 0.0;
+    for (int a = 0; a < n; a++) {
+        for (int d = 0; d < 3; d++) {
+            lo[d] = std::min(lo[d], pos[3 * a + d]);
+            hi[d] = std::max(hi[d], pos[3 * a + d]);
+        }
+        rmax = std::max(rmax, rad[a]);
+    }
+    // the near distance 1.5 (R_s + R_t) and the overlap distance R_s + R_t are
+    // at most 3 R_max: cells of edge (2 + 1) R_max
+    int key_bits = 0;
+    const CellGrid g = cell_grid(n, lo, hi, rmax, 1.0, key_bits);
+    const int64_t ncells = (int64_t)g.dim[0] * g.dim[1] * g.dim[2];
+    Dev<int> idx(n), sorted(n), begin(ncells), end(ncells), overlap(n);
+    Dev<unsigned> key(n), skey(n);
+    Dev<int64_t> count(n + 1), off(n + 1);
+    k_cell_keys<<<grid_of(n), 256, 0, s>>>(n, dpos, g, key.p, idx.p);
+    size_t tmp_sort = 0, tmp_scan = 0;
+    SPHSTK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_sort, key.p, skey.p, idx.p, sorted.p, n, 0, key_bits,
+                                                  s));
+    SPHSTK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_scan, count.p, off.p, n + 1, s));
+    Dev<unsigned char> tmp(std::max(tmp_sort, tmp_scan));
+    SPHSTK_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_sort, key.p, skey.p, idx.p, sorted.p, n, 0, key_bits, s));
+    const unsigned cg = (unsigned)((ncells + 255) / 256);
+    k_fill_i32<<<cg, 256, 0, s>>>(ncells, 0, begin.p);
+    k_fill_i32<<<cg, 256, 0, s>>>(ncells, 0, end.p);
+    k_fill_i64<<<1, 64, 0, s>>>(1, 0, count.p + n);
+    k_cell_bounds<<<grid_of(n), 256, 0, s>>>(n, skey.p, begin.p, end.p);
+    k_near<false><<<grid_of64(n), 64, 0, s>>>(n, dpos, drad, dsusp, g, sorted.p, begin.p, end.p, count.p, overlap.p,
+                                              nullptr, nullptr);
+    SPHSTK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_scan, count.p, off.p, n + 1, s));
+    SPHSTK_HIP(hipGetLastError());
+    std::vector<int> ov(n);
+    SPHSTK_HIP(hipMemcpyAsync(near_off.data(), off.p, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s));
+    SPHSTK_HIP(hipMemcpyAsync(ov.data(), overlap.p, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    SPHSTK_HIP(hipStreamSynchronize(s));
+    for (int t = 0; t < n; t++)
+        SPHSTK_REQUIRE(ov[t] < 0, "overlapping spheres " + std::to_string(t) + " and " + std::to_string(ov[t]) +
+                                      " (centre distance < R_s + R_t)");
+    const int64_t total = near_off[n];
+    near_src.resize(total);
+    if (total == 0) return;
+    Dev<int> dsrc(total);
+    k_near<true><<<grid_of64(n), 64, 0, s>>>(n, dpos, drad, dsusp, g, sorted.p, begin.p, end.p, nullptr, nullptr, off.p,
+                                             dsrc.p);
+    SPHSTK_HIP(hipGetLastError());
+    SPHSTK_HIP(hipMemcpyAsync(near_src.data(), dsrc.p, sizeof(int) * total, hipMemcpyDeviceToHost, s));
+    SPHSTK_HIP(hipStreamSynchronize(s));
 }
 
 int64_t collision_pairs_noalloc(int n, const double *dpos, const double *drc, const int *dgid, double colbuf,

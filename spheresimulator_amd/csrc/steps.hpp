@@ -17,6 +17,14 @@ void brownian_velocity(int n, const double *radius, double mu, double kBT, doubl
 int64_t collision_pairs(int n, const double *pos, const double *rcol, const int *gid, double colbuf,
                         int64_t capacity, int *ii, int *jj, double *phi0, double *norm);
 
+// Near pairs of the boundary-integral operator on the device: (t, s) in one
+// suspension (susp per sphere) with 0 < |c_s - c_t|^2 < (1.5 R_s + 1.5 R_t)^2,
+// grouped by t with s ascending, returned as CSR on the host (pos/rad: host
+// copies of the device arrays dpos/drad, for the cell grid).  Throws when two
+// spheres of a suspension overlap (centre distance < R_s + R_t).
+void near_pairs(int n, const double *pos, const double *rad, const double *dpos, const double *drad, const int *dsusp,
+                hipStream_t s, std::vector<int64_t> &near_off, std::vector<int> &near_src);
+
 // The same search on device arrays with no device allocation: every
 // intermediate array lives in a static device scratch sized for
 // kNoAllocMaxSpheres spheres; gid may be NULL (the index); the four outputs

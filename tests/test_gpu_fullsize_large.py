@@ -240,10 +240,16 @@ def cfg5():
     return dict(c=c, ref=ref, norms=norms.ravel(), w=w, off=off, sample=sample)
 
 
-def test_cfg5_fp64_apply_parity_tight_pairs(cfg5):
+def test_cfg5_near_list_and_fp64_apply_parity_tight_pairs(cfg5):
+    """The device near-pair list equals the oracle's (85,524 ordered pairs), and
+    the FP64 apply matches it on the spheres of the tightest pairs."""
     c, ref, off, sample = cfg5["c"], cfg5["ref"], cfg5["off"], cfg5["sample"]
-    say("16384 spheres p=12: FP64 apply on the spheres of the tightest pairs")
+    say("16384 spheres p=12: near list; FP64 apply on the spheres of the tightest pairs")
     gpu = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+    t, s = gpu.near_pairs()
+    tr, sr = ref.near_pairs()
+    print("near pairs: %d on the device, %d in the oracle" % (len(t), len(tr)))
+    assert len(t) > 80000 and np.array_equal(t, tr) and np.array_equal(s, sr)
     x = np.random.default_rng(51).standard_normal(ref.dof)
     rows = rows_of(off, sample)
     err = rel(gpu.apply(x)[rows], sampled(ref, x, sample, 0.5, 0.0, 1.0, 1.0)[rows])

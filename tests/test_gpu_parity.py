@@ -78,6 +78,32 @@ def test_grid_and_near_list_match(geo):
     assert np.array_equal(t, tr) and np.array_equal(s, sr)
 
 
+def test_near_list_batched_and_overlap_rejected():
+    """The device near-pair search over a batch whose suspensions occupy the same
+    region of space: pairs only within a suspension, each suspension's list equal
+    to the oracle's for that suspension alone (same order), and two overlapping
+    spheres of one suspension rejected at creation."""
+    say("near list: batched suspensions in one region, overlap rejected")
+    c = S.configs.config4(n_susp=12)
+    gpu = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], counts=c["counts"])
+    t, s = gpu.near_pairs()
+    off = np.concatenate([[0], np.cumsum(c["counts"])])
+    want_t, want_s = [], []
+    for b in range(len(c["counts"])):
+        sl = slice(off[b], off[b + 1])
+        tr, sr = O.Operator(c["pos"][sl], c["radius"][sl], c["quat"][sl], c["order"][sl]).near_pairs()
+        want_t.append(tr + off[b])
+        want_s.append(sr + off[b])
+    assert len(t) > 100
+    assert np.array_equal(t, np.concatenate(want_t)) and np.array_equal(s, np.concatenate(want_s))
+    pos = np.array([[0.0, 0.0, 0.0], [5.0, 0.0, 0.0], [6.9, 0.0, 0.0]])
+    with pytest.raises(S.SphstkError, match="overlapping spheres 1 and 2"):
+        S.SphereSTKSHOperator(pos, [1.0, 1.0, 1.0], None, 4)
+    # the same three spheres split over two suspensions do not interact
+    ok = S.SphereSTKSHOperator(pos, [1.0, 1.0, 1.0], None, 4, counts=[2, 1])
+    assert ok.n_near_pairs == 0
+
+
 def test_apply_lop_project_parity(geo):
     say("apply/lop/project")
     pos, rad, quat, order = geo
@@ -242,6 +268,41 @@ def test_inner_fp32_operator_accuracy(p, direct, tol):
     err = rel(y32, y64)
     print("inner rel err %.3e" % err)
     assert err < tol
+
+
+def test_inner_operator_source_cells():
+    """Source-cell aggregation of the FP32 inner operator's two coarsest levels
+    (cells of 16 Morton-consecutive spheres as 200 equivalent point forces,
+    used for targets beyond 4 cell radii): on a dense suspension where cells
+    are in use it evaluates fewer pairs, and the operator stays as close to the
+    FP64 direct-sum operator as without cells (band-limited and white
+    densities), and to the oracle's direct sums on sampled spheres."""
+    say("inner FP32 operator with source cells (4096 spheres p=8 vf 0.2)")
+    c = S.configs.make_suspension(4096, 8, 37, vf=0.2, min_gap=0.1)
+    args = (c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+    with_cells = S.SphereSTKSHOperator(*args)
+    without = S.SphereSTKSHOperator(*args, cells=False)
+    white = np.random.default_rng(9).standard_normal(with_cells.n_dof)
+    band = with_cells.projectNullSpace(white.copy())
+    for name, x in (("band-limited", band), ("white", white)):
+        y64 = with_cells.apply(x)
+        e_cells, e_plain = rel(with_cells.apply_inner(x), y64), rel(without.apply_inner(x), y64)
+        print("%s: inner rel err with cells %.3e, without %.3e" % (name, e_cells, e_plain))
+        assert e_cells < 1e-5 and e_cells < 1.5 * e_plain + 2e-6
+    p_cells, p_plain = with_cells.kernel_stats(0, 1)["pairs"], without.kernel_stats(0, 1)["pairs"]
+    print("pairs per inner apply: with cells %.4g, without %.4g" % (p_cells, p_plain))
+    assert p_cells < 0.97 * p_plain
+    ref = O.Operator(*args)
+    _, _, _, off = ref.grid()
+    sample = (0, 1500, 4095)
+    rows = np.concatenate([np.arange(3 * off[i], 3 * off[i + 1]) for i in sample])
+    want = np.zeros(ref.dof)
+    for i in sample:
+        ref.apply_p2p_spheres(band, i, i + 1, want, 0.5, 0.0, 1.0)
+    ref.apply_lop(band, want, 1.0)
+    err = rel(with_cells.apply_inner(band)[rows], want[rows])
+    print("inner with cells vs oracle on sampled spheres %.3e" % err)
+    assert err < 1e-5
 
 
 def test_inner_level_orders_from_environment(monkeypatch):
