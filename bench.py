@@ -1,17 +1,31 @@
 """Benchmark: mobility solves/sec to GMRES 1e-8 on one MI355X.
 
-A step is one full mobility solve (SphereSTKMobMat::solveMob,
-BIE/SphereSTKMobMat.cpp:146-286): RHS traction apply, GMRES(30) to
-||r||/||b|| <= 1e-8 with the explicit residual check, the final single-layer
-apply and the rigid-body averages, from a cold start (x0 = 0) with the inputs
-resident in device memory.  Both forms are timed on the same workload: the FP64
-plain form and the mixed form (FP32 far field inside FP64 iterative
-refinement); `value` is the mixed form's solves/sec, the FP64 figure, their
-ratio and the mixed form's velocity error against the FP64 form are reported
-beside it.  The far-field traction kernel's roofline comes from HIP events
-recorded by the library on its own stream.  The CPU baseline is the oracle's
-FP64 far field (the dominant cost) on a bounded sample of targets, scaled to one
-solve with the GPU-measured iteration count.
+Workload: the 16384-sphere, p=12 suspension (lognormal radii, vf 0.2, chains
+of 8 with close gaps; configs.config5, seed 5), the size the target is quoted
+at.  A step is one full mobility solve (SphereSTKMobMat::solveMob,
+BIE/SphereSTKMobMat.cpp:146-286) from a cold start (x0 = 0) with the inputs
+resident in device memory: the right-hand side, GMRES to ||r||/||b|| <= 1e-8,
+the final single layer and the rigid-body averages.
+
+* value: the mixed form's solves/sec (FP32 multi-level inner operator inside
+  FP64 iterative refinement).
+* FP64 plain form: one full FP64 operator apply is timed here; its solve time
+  is that apply time x the applies of a completed FP64 solve of this workload
+  (tests/golden/cfg5_fp64_reference.npz, made by
+  tests/golden/make_cfg5_fp64_reference.py in two GPU jobs), labelled
+  "extrapolated"; the fixture's own two-job wall-clock is reported beside it.
+* Error: the mixed velocities against that FP64 solution.
+* roofline: the mixed form's dominant kernel, k_far_f32 (the seven lists of
+  one FP32 inner apply: mid range, four levels, two source-cell lists), from
+  HIP events the library records on its own stream around those launches.
+* cpu_baseline: the oracle's full operator apply (far, self and near terms)
+  on a bounded sample of target spheres, scaled to a full FP64 solve with the
+  FP64 apply count (kind "port", all host threads), labelled "extrapolated".
+* The 1024-sphere and batched workloads follow as extra keys when time allows.
+
+The driver runs `--steps 20 --warmup 5` under a time limit: when the first
+solve shows that the requested steps would not fit `--budget` seconds, the
+warmup is cut first and then the steps, and the line says so (`requested`).
 """
 import argparse
 import json
@@ -27,66 +41,135 @@ sys.path.insert(0, ROOT)
 FP64_VALU_PEAK = 78.6  # TFLOP/s, MI355X FP64 vector (datasheet)
 FP32_VALU_PEAK = 157.3  # TFLOP/s, MI355X FP32 vector (datasheet; counts packed FP32, which this build does not emit)
 FLOPS_TRAC = 30  # algorithmic flops per far traction pair
-FLOPS_SL = 29
+REF_FIXTURE = os.path.join(ROOT, "tests", "golden", "cfg5_fp64_reference.npz")
+T0 = time.perf_counter()
 
 
 def log(msg):
-    print("[bench] " + msg, file=sys.stderr, flush=True)
+    print("[bench %6.1f s] %s" % (time.perf_counter() - T0, msg), file=sys.stderr, flush=True)
 
 
-def run_form(S, c, precision, steps, warmup):
-    t0 = time.perf_counter()
-    M = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], counts=c["counts"], precision=precision,
-                          timing=True)
-    setup = time.perf_counter() - t0
-    zero = np.zeros(M.op.n_dof)
-    info = None
-    for _ in range(warmup):
-        M.set_unknown(zero)
-        v, info = M.solveMob(c["force"])
-    M.op.reset_stats()
-    times = []
-    for _ in range(steps):
+def rel_err(v, ref):
+    v, ref = v.reshape(-1, 6), ref.reshape(-1, 6)
+    return (float(np.linalg.norm(v[:, :3] - ref[:, :3]) / np.linalg.norm(ref[:, :3])),
+            float(np.linalg.norm(v[:, 3:] - ref[:, 3:]) / np.linalg.norm(ref[:, 3:])))
+
+
+def solve_loop(M, force, steps, warmup, budget_end=None, fixed_after=0.0):
+    """warmup untimed solves, then steps timed ones, each from x0 = 0.  With
+    budget_end, the counts are cut (warmup first, at least one timed step) once
+    the first solve's time shows they would not finish fixed_after seconds
+    before it."""
+    zero = np.zeros(M.n_dof)
+    req_steps, req_warm = steps, warmup
+
+    def one():
         M.set_unknown(zero)
         t0 = time.perf_counter()
-        v, info = M.solveMob(c["force"])
-        times.append(time.perf_counter() - t0)
-    stats = {(k, p): M.op.kernel_stats(kind=k, prec=p) for k in range(2) for p in range(3)}
-    return M, v, info, times, stats, setup
+        v, info = M.solveMob(force)
+        return time.perf_counter() - t0, v, info
+
+    times = []
+    if warmup == 0:
+        M.op.reset_stats()
+    t1, v, info = one()
+    warm_done = 1 if warmup > 0 else 0
+    if warmup == 0:
+        times.append(t1)
+    if budget_end is not None:
+        fit = int((budget_end - time.perf_counter() - fixed_after) // (1.02 * t1))
+        rw, rs = warmup - warm_done, steps - len(times)
+        if rw + rs > fit:
+            rw = max(0, min(rw, fit - rs))
+            rs = max(0 if times else 1, min(rs, fit - rw))
+            warmup, steps = warm_done + rw, len(times) + rs
+    for _ in range(warmup - warm_done):
+        one()
+    if warm_done or warmup:
+        M.op.reset_stats()
+    while len(times) < steps:
+        t, v, info = one()
+        times.append(t)
+    return v, info, times, dict(steps=steps, warmup=warmup, requested_steps=req_steps, requested_warmup=req_warm)
 
 
-def pmc_traffic(c, pairs):
-    """HBM bytes per launch of the FP64 traction kernel from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE passes (profiles/), when they cover this workload."""
-    path = os.path.join(ROOT, "profiles", "far_traction_f64_cfg2_pmc.json")
-    if not os.path.exists(path) or not c["name"].startswith("cfg2"):
+def far_roofline(st, peak, kernel, launches_per_apply):
+    """Achieved rate of the far-field traction launches of one operator apply:
+    algorithmic flops (FLOPS_TRAC per source-target pair, all lists of the
+    apply) over the time between the HIP events the library records on its
+    own stream before the first and after the last of those launches."""
+    if not st["launches"]:
         return None
-    with open(path) as f:
-        return json.load(f)["traffic_bytes"]
+    apply_ms = st["ms"] / st["launches"]
+    ach = FLOPS_TRAC * st["pairs"] / (apply_ms / 1000.0) / 1e12
+    return {"bound": "valu", "kernel": kernel, "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+            "frac": ach / peak, "traffic": None, "avg_launch_ms": apply_ms / launches_per_apply,
+            "launches_per_apply": launches_per_apply, "avg_apply_ms": apply_ms,
+            "pairs_per_apply": st["pairs"], "flops_per_pair": FLOPS_TRAC, "applies_timed": st["launches"]}
 
 
-def cpu_baseline(c, iters, budget_s=15.0):
+def cpu_baseline(c, applies, budget_s=8.0):
+    """The oracle's full operator apply (far + self + near) on a sample of
+    target spheres, scaled to all spheres and to `applies` applies per solve."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
-    op = O.Operator(c["pos"], c["radius"], c["quat"], c["order"])
-    x = np.random.default_rng(0).standard_normal(op.dof)
-    n = op.npts
-    k = max(64, min(n, 4096))
+    ref = O.Operator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+    x = np.random.default_rng(0).standard_normal(ref.dof)
+    out = np.zeros(ref.dof)
+    n = len(c["radius"])
     t0 = time.perf_counter()
-    op.far_sums_range(x, 0, k, cTrac=1.0)
+    k = 0
+    stride = max(1, n // 997)
+    while k < n and (time.perf_counter() - t0 < budget_s or k == 0):
+        i = (k * stride) % n
+        ref.apply_p2p_spheres(x, i, i + 1, out, 0.5, 0.0, 1.0)
+        k += 1
     dt = time.perf_counter() - t0
-    # grow the sample to fill roughly budget_s of CPU work
-    k2 = int(min(n, max(k, k * budget_s / max(dt, 1e-6) * 0.5)))
-    if k2 > k:
-        t0 = time.perf_counter()
-        op.far_sums_range(x, 0, k2, cTrac=1.0)
-        dt = time.perf_counter() - t0
-        k = k2
-    t_apply = dt * n / k
-    applies = iters + 3  # RHS + Krylov iterations + explicit residual + final single layer
+    t_apply = dt / k * n
     return dict(value=1.0 / (t_apply * applies), unit="solves/s", cores=O.num_threads(), kind="port",
-                sample="oracle FP64 far-field traction sum for %d of %d targets (%.1f s), scaled to %d applies "
-                       "per solve (direct sum in place of STKFMM)" % (k, n, dt, applies))
+                sample="extrapolated: the oracle's full FP64 apply (direct far sum in place of STKFMM, self and "
+                       "near terms) on %d of %d target spheres (%.1f s), scaled to all spheres (%.1f s per apply) "
+                       "and %d applies per FP64 solve" % (k, n, dt, t_apply, applies))
+
+
+def fp64_apply_time(S, c, budget_left):
+    """Set-up and one timed FP64 apply A64 x (direct far sums) on the workload."""
+    t0 = time.perf_counter()
+    op = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0, timing=True)
+    setup = time.perf_counter() - t0
+    x = np.random.default_rng(1).standard_normal(op.n_dof)
+    t0 = time.perf_counter()
+    op.apply(x)
+    t_apply = time.perf_counter() - t0
+    st = op.kernel_stats(0, 0)
+    return setup, t_apply, st
+
+
+def extra_workload(S, name, c, deadline, mixed_steps=2, fp64_steps=1):
+    """Mixed (one warmup solve) and FP64 (no warmup) solves of a smaller
+    workload, timed, not extrapolated."""
+    res = {}
+    for form, prec, steps, warm in (("mixed", S.PREC_MIXED, mixed_steps, 1), ("fp64", S.PREC_FP64, fp64_steps, 0)):
+        if time.perf_counter() > deadline:
+            res[form] = "skipped: time budget"
+            continue
+        t0 = time.perf_counter()
+        M = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], counts=c["counts"], precision=prec)
+        setup = time.perf_counter() - t0
+        v, info, times, cnt = solve_loop(M, c["force"], steps, warm)
+        res[form] = dict(solves_per_s=1.0 / float(np.mean(times)), s_per_solve=float(np.mean(times)),
+                         setup_s=setup, iters=info["iters"], outer=info["outer"], relres=info["relres"],
+                         steps=cnt["steps"], v=v)
+        log("%s %s: setup %.2f s, %.3f s/solve, iters %d" % (name, form, setup, np.mean(times), info["iters"]))
+        del M
+    if isinstance(res.get("mixed"), dict) and isinstance(res.get("fp64"), dict):
+        eV, eW = rel_err(res["mixed"]["v"], res["fp64"]["v"])
+        res["mixed_vs_fp64"] = dict(speedup=res["mixed"]["solves_per_s"] / res["fp64"]["solves_per_s"],
+                                    rel_err_V=eV, rel_err_Omega=eW)
+    for f in ("mixed", "fp64"):
+        if isinstance(res.get(f), dict):
+            res[f].pop("v")
+    return res
 
 
 def main():
@@ -94,82 +177,80 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--budget", type=float, default=570.0, help="seconds the whole run may take")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--no-extras", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    end = T0 + args.budget
     import spheresimulator_amd as S
-    log("building config %d" % args.config)
-    c = S.configs.CONFIGS[args.config]()
-    res = {}
-    for form in args.forms.split(","):
-        prec = S.PREC_FP64 if form == "fp64" else S.PREC_MIXED
-        log("timing %s form: %d warmup + %d steps" % (form, args.warmup, args.steps))
-        M, v, info, times, stats, setup = run_form(S, c, prec, args.steps, args.warmup)
-        res[form] = dict(v=v, info=info, times=times, stats=stats, n_dof=M.op.n_dof, near=M.op.n_near_pairs,
-                         setup=setup)
-        log("%s: setup %.3f s, %.3f s/solve, iters=%d outer=%d relres=%.2e" % (
-            form, setup, np.mean(times), info["iters"], info["outer"], info["relres"]))
-        del M
-    main_form = "mixed" if "mixed" in res else "fp64"
-    r = res[main_form]
-    ms = 1000.0 * float(np.mean(r["times"]))
+    log("building the 16384-sphere workload")
+    c = S.configs.config5()
+    ref = np.load(REF_FIXTURE, allow_pickle=False)
+    ref_meta = json.loads(str(ref["meta"]))
+    fp64_applies = int(ref_meta["fp64_applies"])
+
+    # mixed form: the headline
+    t0 = time.perf_counter()
+    M = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], precision=S.PREC_MIXED, timing=True)
+    setup_mixed = time.perf_counter() - t0
+    log("mixed set-up %.2f s; solving (%d warmup + %d steps requested)" % (setup_mixed, args.warmup, args.steps))
+    # reserve time for the FP64 apply (~24 s), the CPU sample, the smaller
+    # workloads and writing the line
+    reserve = 32.0 + (0.0 if args.no_cpu else 12.0) + (0.0 if args.no_extras else 42.0)
+    v, info, times, cnt = solve_loop(M, c["force"], args.steps, args.warmup, budget_end=end, fixed_after=reserve)
+    ms = 1000.0 * float(np.mean(times))
+    log("mixed: %.3f s/solve over %d steps (warmup %d), iters %d outer %d relres %.2e" % (
+        ms / 1000, cnt["steps"], cnt["warmup"], info["iters"], info["outer"], info["relres"]))
+    st32 = M.op.kernel_stats(0, 1)
+    st64ml = M.op.kernel_stats(0, 2)
+    n_dof, n_near = M.n_dof, M.op.n_near_pairs
+    del M
+    eV, eW = rel_err(v, ref["vel"])
+
+    log("FP64 form: one timed apply")
+    setup64, t_apply64, st64 = fp64_apply_time(S, c, end - time.perf_counter())
+    t_fp64 = t_apply64 * fp64_applies
+    log("FP64 apply %.2f s -> %.1f s per solve (%d applies)" % (t_apply64, t_fp64, fp64_applies))
+
     out = {"metric": "mobility solves/sec to GMRES 1e-8 (mixed FP32-far-field + FP64 refinement vs FP64), 1 GPU",
-           "value": 1000.0 / ms, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "value": 1000.0 / ms, "unit": "solves/s", "n_gpus": 1, "steps": cnt["steps"], "warmup": cnt["warmup"],
            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f64" if main_form == "fp64" else "f32+f64", "data": "synthetic",
-           "config": {"workload": c["name"], "spheres": int(len(c["radius"])), "sh_order": int(c["order"].max()),
-                      "dof": int(r["n_dof"]), "near_pairs": int(r["near"]), "gmres_tol": 1e-8, "restart": 30,
-                      "form": main_form}}
-    # one-time set-up (geometry, near list, work lists, level grids; the
-    # reference rebuilds its operator every time step) and the rate with it
-    out["setup_s"] = {k: float(v["setup"]) for k, v in res.items()}
-    out["solves_per_s_with_setup"] = {k: 1.0 / (float(v["setup"]) + float(np.mean(v["times"])))
-                                      for k, v in res.items()}
-    if "fp64" in res:
-        f = res["fp64"]
-        out["fp64_solves_per_s"] = 1.0 / float(np.mean(f["times"]))
-        out["fp64_iters"] = f["info"]["iters"]
-        st = f["stats"][(0, 0)]  # traction, FP64
-        if st["launches"]:
-            avg_s = st["ms"] / st["launches"] / 1000.0
-            ach = FLOPS_TRAC * st["pairs"] / avg_s / 1e12
-            out["roofline"] = {"bound": "valu", "kernel": "k_far_f64<traction>", "achieved": ach,
-                               "peak": FP64_VALU_PEAK, "unit": "TFLOP/s", "frac": ach / FP64_VALU_PEAK,
-                               "traffic": pmc_traffic(c, st["pairs"]), "avg_launch_ms": st["ms"] / st["launches"],
-                               "pairs_per_launch": st["pairs"], "flops_per_pair": FLOPS_TRAC}
-    if "mixed" in res and "fp64" in res:
-        vm = res["mixed"]["v"].reshape(-1, 6)
-        vf = res["fp64"]["v"].reshape(-1, 6)
-        out["mixed_vs_fp64"] = {
-            "speedup": float(np.mean(res["fp64"]["times"]) / np.mean(res["mixed"]["times"])),
-            "rel_err_V": float(np.linalg.norm(vm[:, :3] - vf[:, :3]) / np.linalg.norm(vf[:, :3])),
-            "rel_err_Omega": float(np.linalg.norm(vm[:, 3:] - vf[:, 3:]) / np.linalg.norm(vf[:, 3:])),
-            "mixed_iters": res["mixed"]["info"]["iters"], "mixed_outer": res["mixed"]["info"]["outer"],
-            "mixed_fp32_applies": res["mixed"]["info"]["applies_fp32"],
-            "mixed_fp64_applies": res["mixed"]["info"]["applies_fp64"]}
-        st = res["mixed"]["stats"][(0, 1)]  # traction, FP32 inner operator (all levels of one apply)
-        if st["launches"]:
-            avg_s = st["ms"] / st["launches"] / 1000.0
-            ach = FLOPS_TRAC * st["pairs"] / avg_s / 1e12
-            out["mixed_vs_fp64"]["fp32_far"] = {
-                "bound": "valu", "achieved": ach, "peak": FP32_VALU_PEAK, "unit": "TFLOP/s",
-                "frac": ach / FP32_VALU_PEAK, "avg_apply_ms": st["ms"] / st["launches"], "pairs_per_apply": st["pairs"]}
-        st = res["mixed"]["stats"][(0, 2)]  # traction, FP64 multi-level outer applies
-        if st["launches"]:
-            avg_s = st["ms"] / st["launches"] / 1000.0
-            ach = FLOPS_TRAC * st["pairs"] / avg_s / 1e12
-            out["mixed_vs_fp64"]["fp64_outer_far"] = {
-                "bound": "valu", "achieved": ach, "peak": FP64_VALU_PEAK, "unit": "TFLOP/s",
-                "frac": ach / FP64_VALU_PEAK, "avg_apply_ms": st["ms"] / st["launches"], "pairs_per_apply": st["pairs"]}
+           "dtype": "f32+f64", "data": "synthetic",
+           "config": {"workload": "cfg5: 16384 spheres p=12, lognormal radii (sigma 0.2), vf 0.2, chains of 8",
+                      "spheres": int(len(c["radius"])), "sh_order": 12, "dof": int(n_dof),
+                      "near_pairs": int(n_near), "gmres_tol": 1e-8, "restart": 30, "form": "mixed"}}
+    if (cnt["steps"], cnt["warmup"]) != (args.steps, args.warmup):
+        out["requested"] = {"steps": args.steps, "warmup": args.warmup,
+                            "note": "cut to fit the %.0f s budget after the first solve took %.1f s" %
+                                    (args.budget, times[0] if times else ms / 1000)}
+    out["mixed"] = {"iters": info["iters"], "outer": info["outer"], "relres": info["relres"],
+                    "applies_fp32": info["applies_fp32"], "applies_fp64": info["applies_fp64"],
+                    "setup_s": setup_mixed, "solves_per_s_with_setup": 1.0 / (setup_mixed + ms / 1000.0),
+                    "rel_err_V_vs_fp64": eV, "rel_err_Omega_vs_fp64": eW}
+    out["fp64"] = {"solves_per_s": 1.0 / t_fp64, "s_per_solve": t_fp64, "basis": "extrapolated",
+                   "apply_s": t_apply64, "applies_per_solve": fp64_applies, "iters": ref_meta["fp64_iters"],
+                   "setup_s": setup64, "two_job_solve_s": ref_meta["fp64_solve_s"],
+                   "far_traction": far_roofline(st64, FP64_VALU_PEAK, "k_far_f64<traction>", 1)}
+    out["mixed_vs_fp64"] = {"speedup": t_fp64 / (ms / 1000.0), "rel_err_V": eV, "rel_err_Omega": eW}
+    out["roofline"] = far_roofline(st32, FP32_VALU_PEAK, "k_far_f32<traction>: the mid-range list, 4 level lists "
+                                   "and 2 source-cell lists of one FP32 inner apply", 7)
+    out["mixed"]["fp64_outer_far"] = far_roofline(st64ml, FP64_VALU_PEAK, "k_far_f64 (mid-range list) and "
+                                                  "k_far_f64_lvl (4 level lists) <traction> of one FP64 residual "
+                                                  "apply", 5)
     if not args.no_cpu:
-        it = res.get("fp64", r)["info"]["iters"]
         log("CPU baseline (oracle, bounded sample)")
-        out["cpu_baseline"] = cpu_baseline(c, it)
+        out["cpu_baseline"] = cpu_baseline(c, fp64_applies)
+    if not args.no_extras:
+        deadline = end - 5.0
+        if time.perf_counter() + 10 < deadline:
+            out["cfg2"] = extra_workload(S, "cfg2", S.configs.config2(), deadline, 3, 2)
+        if time.perf_counter() + 30 < deadline:
+            out["cfg4"] = extra_workload(S, "cfg4", S.configs.config4(), deadline, 2, 1)
+    out["wall_s"] = time.perf_counter() - T0
     print(json.dumps(out), flush=True)
 
 

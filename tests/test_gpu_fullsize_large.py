@@ -295,3 +295,21 @@ def test_cfg5_mixed_solution_on_tight_pairs(cfg5, cfg5_mixed):
     eV, eW = rel(got[:, :3], want[:, :3]), rel(got[:, 3:], want[:, 3:])
     print("mixed velocities vs the oracle's averages of S[rho]: V %.3e Omega %.3e" % (eV, eW))
     assert eV < 1e-9 and eW < 1e-9
+
+
+def test_cfg5_mixed_velocities_against_completed_fp64_solve(cfg5_mixed):
+    """The mixed velocities against the completed FP64 solve of this workload
+    (tests/golden/cfg5_fp64_reference.npz: GMRES(30) to 1e-8 with the direct
+    far sums, made by tests/golden/make_cfg5_fp64_reference.py): the target's
+    error bar, 1e-7 in V and Omega."""
+    import json
+    import os
+    from conftest import GOLDEN
+    ref = np.load(os.path.join(GOLDEN, "cfg5_fp64_reference.npz"), allow_pickle=False)
+    meta = json.loads(str(ref["meta"]))
+    vr = ref["vel"].reshape(-1, 6)
+    v = cfg5_mixed["v"].reshape(-1, 6)
+    eV, eW = rel(v[:, :3], vr[:, :3]), rel(v[:, 3:], vr[:, 3:])
+    print("mixed vs the FP64 solve (%d iterations, relres %.2e): V %.3e Omega %.3e" %
+          (meta["fp64_iters"], meta["fp64_relres"], eV, eW))
+    assert eV < 1e-7 and eW < 1e-7
