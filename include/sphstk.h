@@ -56,6 +56,10 @@ extern "C" {
 #define SPHSTK_FLAG_INNER_PRECOND 16 /* mixed form: right-precondition the inner solve by the isolated-sphere
                                         inverse (sphstk_operator_apply_self_inverse) */
 #define SPHSTK_FLAG_NO_CELLS 32   /* FP32 inner operator: no source-cell aggregation on its two coarsest levels */
+#define SPHSTK_FLAG_EXPLICIT_REFINE 64 /* mixed form: an explicit FP64 residual at every refinement step (default:
+                                          once twice the inner operator's measured relative error times the
+                                          correction's size is below tol / 2, a step takes the inner solve's
+                                          residual plus that bound) */
 
 typedef struct sphstk_operator sphstk_operator;
 typedef struct sphstk_mobility sphstk_mobility;
@@ -79,7 +83,9 @@ typedef struct {
                                form: A is the FP64 operator with the direct far-field sum.  Mixed form:
                                A is the multi-level FP64 operator of sphstk_operator_apply_outer (about
                                1e-9 from the direct-sum operator), b its right-hand side with distant
-                               spheres' rho0 in closed form */
+                               spheres' rho0 in closed form; after a refinement step without an FP64
+                               apply (see SPHSTK_FLAG_EXPLICIT_REFINE) it is the bound
+                               (|inner residual| + measured inner-operator error) / |b| */
     int applies_fp64;       /* FP64 operator applies, including the RHS and final single-layer applies */
     int applies_fp32;       /* FP32 operator applies (mixed precision) */
 } sphstk_solve_info;

@@ -26,6 +26,7 @@ namespace {
 constexpr int kMaxSupportedOrder = 20; // LDS of the expansion kernel bounds the order
 constexpr int64_t kChunk = 4096;  // DOF per block of the segmented vector kernels
 constexpr int64_t kWantItems = 40000; // far-field work items wanted when a problem is small
+constexpr int64_t kLookaheadDof = 300000; // GMRES below this many unknowns launches one iteration ahead
 constexpr double kTuneBudget = 3.0;    // seconds per autotuning pass (SPHSTK_FLAG_AUTOTUNE)
 
 // SPHSTK_VERBOSE=1: set-up phase times on stderr
@@ -74,8 +75,8 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     c_[1] = cSL;
     c_[2] = cTrac;
     c_[3] = cLOP;
-    SPHSTK_REQUIRE((flags & ~63) == 0, "unknown flags (known: TIMING 1, FP32_DIRECT 2, AUTOTUNE 4, GRAPH 8, "
-                                       "INNER_PRECOND 16, NO_CELLS 32)");
+    SPHSTK_REQUIRE((flags & ~127) == 0, "unknown flags (known: TIMING 1, FP32_DIRECT 2, AUTOTUNE 4, GRAPH 8, "
+                                        "INNER_PRECOND 16, NO_CELLS 32, EXPLICIT_REFINE 64)");
     SPHSTK_REQUIRE(nsusp >= 1, "number of suspensions must be >= 1");
     SPHSTK_REQUIRE(ncopy >= 1 && ncopy <= 64 && nsusp % ncopy == 0, "copies must divide the suspensions");
     SPHSTK_REQUIRE(counts != nullptr, "sphere counts must not be NULL");
@@ -794,7 +795,7 @@ static std::vector<Chunk> make_chunks(int nsusp, const std::vector<int64_t> &sus
 // its angular content on the target like (1/(2s-1))^(n+1); these thresholds put
 // the operator's error near 4e-6 relative (1024 spheres, p=12), well inside what
 // two refinement steps against the FP64 residual need.  Level point sets are
-// stored in Morton order of the sphere centres (per suspension) so that the
+// stored in k-d order of the sphere centres (per suspension) so that the
 // spatially compact exclusion sets of the last level are few index ranges.
 namespace {
 int level_order(int lvl, int p) {
@@ -805,17 +806,31 @@ int level_order(int lvl, int p) {
     if (lvl == 3) pc = 3;
     return std::min(pc, p);
 }
-uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
-    auto spread = [](uint64_t v) {
-        v &= 0x1fffff;
-        v = (v | v << 32) & 0x1f00000000ffffULL;
-        v = (v | v << 16) & 0x1f0000ff0000ffULL;
-        v = (v | v << 8) & 0x100f00f00f00f00fULL;
-        v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
-        v = (v | v << 2) & 0x1249249249249249ULL;
-        return v;
-    };
-    return spread(x) | spread(y) << 1 | spread(z) << 2;
+// Balanced k-d order of the spheres idx[b, e): split at a multiple of `leaf`
+// along the longest extent of the centres, the lower half first, down to
+// ranges of at most `leaf` spheres.  Every leaf starts at b + a multiple of
+// leaf, so the leaves are the consecutive runs of `leaf` ranks; they are
+// compacter than runs of a Morton order (the 16384-sphere packing at vf 0.2:
+// median cell radius 6.6 R against 8.2 R).
+void kd_order(const double *pos, int *idx, int b, int e, int leaf) {
+    const int n = e - b;
+    if (n <= leaf) return;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = b; i < e; i++)
+        for (int c = 0; c < 3; c++) {
+            lo[c] = std::min(lo[c], pos[3 * idx[i] + c]);
+            hi[c] = std::max(hi[c], pos[3 * idx[i] + c]);
+        }
+    int ax = 0;
+    for (int c = 1; c < 3; c++)
+        if (hi[c] - lo[c] > hi[ax] - lo[ax]) ax = c;
+    const int h = n < 2 * leaf ? leaf : (n / leaf / 2) * leaf;
+    std::nth_element(idx + b, idx + b + h, idx + e, [&](int u, int v) {
+        const double pu = pos[3 * u + ax], pv = pos[3 * v + ax];
+        return pu < pv || (pu == pv && u < v);
+    });
+    kd_order(pos, idx, b, b + h, leaf);
+    kd_order(pos, idx, b + h, e, leaf);
 }
 } // namespace
 
@@ -824,34 +839,16 @@ void Operator::ensure_levels(LevelSet &S) {
     PhaseTimer pt(S.fp64 ? "FP64 outer level set" : "FP32 inner level set");
     const double *a = S.alo;
     auto lorder = [&](int l, int p) { return S.order[l] < 0 ? level_order(l, p) : std::min(p, S.order[l]); };
-    // Morton order of the spheres within each suspension
+    // k-d order of the spheres within each suspension
     std::vector<int> perm(nsph_), rank(nsph_);
-    for (int b = 0; b < nsusp_; b++) {
-        const int s0 = (int)susp_sph_off_[b], s1 = (int)susp_sph_off_[b + 1];
-        if (s1 <= s0) continue;
-        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-        for (int i = s0; i < s1; i++)
-            for (int c = 0; c < 3; c++) {
-                lo[c] = std::min(lo[c], pos_[3 * i + c]);
-                hi[c] = std::max(hi[c], pos_[3 * i + c]);
-            }
-        const double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
-        std::vector<std::pair<uint64_t, int>> key;
-        for (int i = s0; i < s1; i++) {
-            uint32_t q[3];
-            for (int c = 0; c < 3; c++) q[c] = (uint32_t)std::min(2097151.0, (pos_[3 * i + c] - lo[c]) / ext * 2097151.0);
-            key.push_back({morton3(q[0], q[1], q[2]), i});
-        }
-        std::sort(key.begin(), key.end());
-        for (int k = 0; k < s1 - s0; k++) {
-            perm[s0 + k] = key[k].second;
-            rank[key[k].second] = s0 + k;
-        }
-    }
+    for (int i = 0; i < nsph_; i++) perm[i] = i;
+    for (int b = 0; b < nsusp_; b++)
+        kd_order(pos_.data(), perm.data(), (int)susp_sph_off_[b], (int)susp_sph_off_[b + 1], kCellSize);
+    for (int k = 0; k < nsph_; k++) rank[perm[k]] = k;
     // Pair classes by one scan per target sphere over its suspension's spheres
-    // in Morton order: near (< 1.5 (R_s + R_t), the operator's near-pair
+    // in k-d order: near (< 1.5 (R_s + R_t), the operator's near-pair
     // predicate) and self are skipped, mid (< a0) is summed on the fine grid,
-    // level l covers [a_l, a_l+1).  Level runs are in Morton rank space, the
+    // level l covers [a_l, a_l+1).  Level runs are in k-d rank space, the
     // mid runs in sphere index space.
     auto tn = std::make_unique<PhaseTimer>("  level pair classes");
     std::vector<Runs> mid(nsph_);
@@ -862,7 +859,7 @@ void Operator::ensure_levels(LevelSet &S) {
             sph_s0[t] = (int)susp_sph_off_[b];
             sph_s1[t] = (int)susp_sph_off_[b + 1];
         }
-    // cells of the FP32 set: kCellSize Morton-consecutive spheres of one
+    // cells of the FP32 set: the k-d leaves (kCellSize consecutive ranks) of one
     // suspension, with centre (mean of the member centres) and radius (largest
     // member distance + member radius)
     std::vector<int> cell_r0, cell_r1;
@@ -893,9 +890,12 @@ void Operator::ensure_levels(LevelSet &S) {
         }
     }
     const int ncell = (int)cell_r0.size();
-    // cell runs of each target (by Morton rank) on the level-2 and level-3 grids
+    // cell runs of each target (by k-d rank) on the level-2 and level-3 grids,
+    // and per target the pair evaluations its cell interactions save
     std::vector<std::array<Runs, 2>> crun(S.cells ? nsph_ : 0);
-    parallel_for(nsph_, [&](int ta, int tb) {
+    std::vector<double> saved(S.cells ? nsph_ : 0, 0.0);
+    const int Keq = equiv_tables().K;
+    auto scan = [&](int ta, int tb) {
         std::vector<int> ml;
         // pair class of (t, s): -3 self, -2 near, -1 mid, else the level.  The
         // FP32 set uses the separation s = d / (R_s + R_t).  The FP64 set, whose
@@ -943,9 +943,17 @@ void Operator::ensure_levels(LevelSet &S) {
                                  ez = ccen[3 * c + 2] - pos_[3 * t + 2];
                     if (std::sqrt(ex * ex + ey * ey + ez * ez) >= kCellTheta * crho[c] + rad_[t]) {
                         int lmin = kNumLevels;
-                        for (int r = cell_r0[c]; r < cell_r1[c] && lmin >= 2; r++) lmin = std::min(lmin, klass(t, perm[r]));
+                        double member_pairs = 0.0;
+                        for (int r = cell_r0[c]; r < cell_r1[c] && lmin >= 2; r++) {
+                            const int sm = perm[r], lm = klass(t, sm);
+                            lmin = std::min(lmin, lm);
+                            if (lm >= 0)
+                                member_pairs += double(grid_points(lorder(lm, order_[sm]))) *
+                                                grid_points(lorder(lm, order_[t]));
+                        }
                         if (lmin >= 2) {
                             run_append(crun[rt][lmin - 2], (int)c);
+                            saved[t] += member_pairs - double(Keq) * grid_points(lorder(lmin, order_[t]));
                             continue;
                         }
                     }
@@ -955,7 +963,34 @@ void Operator::ensure_levels(LevelSet &S) {
             std::sort(ml.begin(), ml.end());
             mid[t] = runs_of(ml, s0, s1, true);
         }
-    });
+    };
+    parallel_for(nsph_, scan);
+    if (S.cells) {
+        // keep the cells only where they pay: the pair evaluations they save
+        // against the check velocities of the cells in use (Kc check points x
+        // the members' fine grids, plus the fit, ~K Kc / 5 pair evaluations)
+        double gain = 0.0, cost = 0.0;
+        for (double v : saved) gain += v;
+        std::vector<char> used(ncell, 0);
+        for (const auto &cr : crun)
+            for (const Runs &rr : cr)
+                for (const auto &r : rr)
+                    for (int c = r.first; c < r.second; c++) used[c] = 1;
+        const EquivTables &E = equiv_tables();
+        for (int c = 0; c < ncell; c++)
+            if (used[c]) {
+                double pts = 0.0;
+                for (int r = cell_r0[c]; r < cell_r1[c]; r++) pts += grid_points(order_[perm[r]]);
+                cost += E.Kc * pts + E.K * E.Kc / 5.0;
+            }
+        if (gain < 3.0 * cost) {
+            S.cells = false;
+            crun.clear();
+            for (auto &lv : lvl)
+                for (Runs &r : lv) r.clear();
+            parallel_for(nsph_, scan);
+        }
+    }
     tn.reset();
     {
         PhaseTimer pm("  mid-range lists");
@@ -1010,7 +1045,14 @@ void Operator::ensure_levels(LevelSet &S) {
                 L->up.M[p] = L->d_up[p].p;
                 L->up.nin[p] = rs.nc;
                 L->up.nout[p] = rs.n;
+                if (!S.fp64) {
+                    L->d_down32[p].upload(std::vector<float>(rs.down.begin(), rs.down.end()));
+                    L->d_up32[p].upload(std::vector<float>(rs.up.begin(), rs.up.end()));
+                    L->down.M32[p] = L->d_down32[p].p;
+                    L->up.M32[p] = L->d_up32[p].p;
+                }
             }
+        L->down.fp32 = L->up.fp32 = !S.fp64;
         PhaseTimer pl("  level lists");
         build_far_lists(L->lists, nsusp_, susp_sph_off_, off, off, lvl[l], S.tpl, 0, kWantItems);
         std::vector<int> dummy;
@@ -1684,6 +1726,7 @@ Mobility::Mobility(int nsusp, const int *counts, const double *pos, const double
     : mu_(viscosity), prm_(prm), prec_(precision) {
     inner_restart_ = std::max(prm.restart, 60);
     inner_precond_ = (flags & 16) != 0;
+    explicit_refine_ = (flags & 64) != 0;
     SPHSTK_REQUIRE(std::isfinite(viscosity) && viscosity > 0, "viscosity must be > 0");
     SPHSTK_REQUIRE(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (mixed)");
     if (const char *v = std::getenv("SPHSTK_VERBOSE")) verbose_ = std::atoi(v) != 0;
@@ -1734,7 +1777,7 @@ void Mobility::seg_norms(const double *v, std::vector<double> &out) {
 // Krylov space, Hessenberg matrix and Givens rotations; converged suspensions
 // drop out of the operator applies through the device `active` mask.
 void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int prec, std::vector<int> &act,
-                     std::vector<SegInfo> &info, int &applies) {
+                     std::vector<SegInfo> &info, int &applies, double *res) {
     Operator &A = *A_;
     hipStream_t st = A.stream();
     const int B = A.nsusp();
@@ -1743,7 +1786,7 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
     if (V_.n < size_t(m + 1) * n) V_.alloc(size_t(m + 1) * n);
     h_.alloc(size_t(2 * (m + 1) + 1) * B);
     if (prec == 1 && m + 1 <= 64) mdot_.alloc(size_t(m + 1) * std::max(A.nchunks(), 1));
-    coefv_.alloc(size_t(m) * B);
+    coefv_.alloc(size_t(m + 1) * B);
     const Chunk *ch = A.chunks();
     const int nch = A.nchunks();
     const int *sco = A.seg_chunk_off();
@@ -1776,6 +1819,7 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
         applies++;
         launch_axpby(n, 1.0, b, -1.0, r_.p, st);
     }
+    if (res) launch_axpby(n, 1.0, r_.p, 0.0, res, st);
     seg_norms(r_.p, beta);
     for (int s = 0; s < B; s++)
         if (act[s]) {
@@ -1802,7 +1846,6 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
     d_kdim_.alloc(B);
     d_impok_.alloc(B);
     SPHSTK_HIP(hipMemcpyAsync(d_bn_.p, bn.data(), sizeof(double) * B, hipMemcpyHostToDevice, st));
-    last_kmax_ = 0;
     std::vector<double> inv(B), rel(B);
     std::vector<int> kdim(B), impok(B);
     int total = 0;
@@ -1871,17 +1914,21 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
                 if (nact_host_[j & 1] == 0) break;
             }
             if (total >= prm_.max_iters) break;
-            // one iteration of lookahead: stop once the previous iteration left none active
-            if (j >= 1) {
+            // Large problems wait for this iteration's count before launching the
+            // next (an iteration is milliseconds; the wait costs tens of
+            // microseconds).  Small ones keep one iteration of lookahead, so the
+            // host's turnaround never idles the GPU, at the price of one
+            // iteration launched after the last one that was needed.
+            if (n >= kLookaheadDof) {
+                SPHSTK_HIP(hipEventSynchronize(nact_ev_[j & 1]));
+                if (nact_host_[j & 1] == 0) break;
+            } else if (j >= 1) {
                 SPHSTK_HIP(hipEventSynchronize(nact_ev_[(j - 1) & 1]));
                 if (nact_host_[(j - 1) & 1] == 0) break;
             }
         }
         // x += V y (back substitution of the rotated Hessenberg systems), over
         // the suspensions of this cycle
-        last_kmax_ = kmax;
-        last_m_ = m;
-        last_prec_ = prec;
         launch_gmres_backsub(B, m, d_H_.p, d_g_.p, d_kdim_.p, d_cyc_.p, coefv_.p, st);
         SPHSTK_HIP(hipMemcpyAsync(act_.p, d_cyc_.p, sizeof(int) * B, hipMemcpyDeviceToDevice, st));
         if (rprec) {
@@ -1892,6 +1939,12 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
             launch_axpby(n, 1.0, z_.p, 1.0, x, st);
         } else {
             launch_seg_multi_axpy(x, V_.p, n, coefv_.p, kmax, B, ch, nch, act_.p, st);
+        }
+        if (res) {
+            // res = V_{k+1} Omega^T g_k e_k, the cycle's final residual (for the
+            // suspensions of this cycle; the others keep theirs)
+            launch_gmres_resid_coef(B, m, d_g_.p, d_cs_.p, d_sn_.p, d_kdim_.p, d_cyc_.p, coefv_.p, st);
+            launch_seg_multi_comb(res, V_.p, n, coefv_.p, kmax + 1, B, ch, nch, act_.p, st);
         }
         SPHSTK_HIP(hipMemcpyAsync(kdim.data(), d_kdim_.p, sizeof(int) * B, hipMemcpyDeviceToHost, st));
         SPHSTK_HIP(hipMemcpyAsync(impok.data(), d_impok_.p, sizeof(int) * B, hipMemcpyDeviceToHost, st));
@@ -1979,8 +2032,19 @@ void Mobility::solve(const double *force, double *vel, std::vector<SegInfo> &inf
         gmres(b_.p, x_.p, x_zero_, prm_.tol, 0, act, info, app64);
         outer = 1;
     } else {
-        // iterative refinement: r = b - A64 x; inner FP32-far-field GMRES on A d = r; x += d
-        std::vector<double> rn;
+        // iterative refinement: r = b - A64 x; inner FP32-far-field GMRES on A d = r; x += d.
+        // A step's new residual is r - A64 d.  The inner solve returns its own
+        // residual rin = r - A32 d (from its basis, no apply), so
+        //   r - A64 d = rin + (A32 - A64) d,
+        // and |(A32 - A64) d| ~ eps |A32 d| with eps the inner operator's
+        // relative error, measured per suspension at every explicit step as
+        // |r_explicit - rin| / |A32 d|.  Once the corrections are small, that
+        // term is below the tolerance: a step whose bound, with eps doubled
+        // for a correction whose content differs from the measured one, stays
+        // under tol / 2 takes rin as the residual, adds the bound to a running
+        // error bound, and skips the FP64 apply.  relres is then
+        // (|r| + bound) / |b|.
+        std::vector<double> rn, eps(B, -1.0), errb(B, 0.0), rinn, a32n, dn;
         if (x_zero_) {
             launch_axpby(n, 1.0, b_.p, 0.0, r_.p, st);
         } else {
@@ -1991,6 +2055,7 @@ void Mobility::solve(const double *force, double *vel, std::vector<SegInfo> &inf
         seg_norms(r_.p, rn);
         for (int s = 0; s < B; s++)
             if (act[s]) info[s].relres = rn[s] / bn[s];
+        rin_.alloc(n);
         while (outer < prm_.max_refine) {
             std::vector<int> a2(B, 0);
             bool anyact = false;
@@ -2016,34 +2081,63 @@ void Mobility::solve(const double *force, double *vel, std::vector<SegInfo> &inf
             // the inner solve overwrites r_, so its right-hand side is a copy of the outer residual
             gmres_inner_rhs_.alloc(n);
             launch_axpby(n, 1.0, r_.p, 0.0, gmres_inner_rhs_.p, st);
-            // from the second step on, the initial correction is the minimum-
-            // residual one within the previous step's last Krylov space (same
-            // operator: A32 V_k = V_k+1 Hbar): it removes the previous inner
-            // residual, which lies in that space, from the new right-hand side
-            bool warm = false;
-            if (outer > 0 && last_prec_ == 1 && last_kmax_ > 0 && !inner_precond_ && last_kmax_ + 1 <= 64) {
-                const int k = last_kmax_;
-                SPHSTK_HIP(hipMemcpyAsync(act_.p, a2.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
-                launch_seg_multi_dot(gmres_inner_rhs_.p, V_.p, n, k + 1, A.chunks(), A.nchunks(), mdot_.p, act_.p, st);
-                launch_seg_multi_reduce(mdot_.p, A.nchunks(), A.seg_chunk_off(), B, k + 1, h_.p, 0, act_.p, st);
-                launch_gmres_project(B, last_m_, h_.p, d_H_.p, d_cs_.p, d_sn_.p, d_kdim_.p, act_.p, coefv_.p, st);
-                launch_seg_multi_axpy(d_.p, V_.p, n, coefv_.p, k, B, A.chunks(), A.nchunks(), act_.p, st);
-                warm = true;
-            }
-            gmres(gmres_inner_rhs_.p, d_.p, !warm, itol, 1, ia, inner, app32);
+            gmres(gmres_inner_rhs_.p, d_.p, true, itol, 1, ia, inner, app32, rin_.p);
             SPHSTK_HIP(hipMemcpyAsync(act_.p, a2.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
             std::vector<double> ones(B, 1.0);
             SPHSTK_HIP(hipMemcpyAsync(scal_.p + 2 * B, ones.data(), sizeof(double) * B, hipMemcpyHostToDevice, st));
             launch_seg_axpy(x_.p, d_.p, scal_.p + 2 * B, 1.0, A.chunks(), A.nchunks(), act_.p, st);
-            A.apply_into(x_.p, r_.p, p64, act_.p);
-            app64++;
-            launch_axpby(n, 1.0, b_.p, -1.0, r_.p, st);
-            seg_norms(r_.p, rn);
+            // A32 d = r - rin (into the right-hand side's copy) and the norms
+            launch_axpby(n, -1.0, rin_.p, 1.0, gmres_inner_rhs_.p, st);
+            seg_norms(rin_.p, rinn);
+            seg_norms(gmres_inner_rhs_.p, a32n);
+            bool update = !explicit_refine_;
             for (int s = 0; s < B; s++)
-                if (a2[s]) {
-                    info[s].iters += inner[s].iters;
-                    info[s].relres = rn[s] / bn[s];
+                if (a2[s] && !(eps[s] >= 0.0 && errb[s] + 2.0 * eps[s] * a32n[s] <= 0.5 * prm_.tol * bn[s]))
+                    update = false;
+            if (verbose_) {
+                double we = -1.0, wb = 0.0;
+                for (int s = 0; s < B; s++)
+                    if (a2[s]) {
+                        we = std::max(we, eps[s]);
+                        wb = std::max(wb, (errb[s] + 2.0 * eps[s] * a32n[s]) / bn[s]);
+                    }
+                std::fprintf(stderr, "[sphstk] refinement step %d: max inner-operator error %.3e, residual bound %.3e: %s\n",
+                             outer + 1, we, wb, update ? "updated residual" : "FP64 residual");
+            }
+            if (update) {
+                launch_axpby(n, 1.0, rin_.p, 0.0, r_.p, st);
+                for (int s = 0; s < B; s++)
+                    if (a2[s]) {
+                        errb[s] += 2.0 * eps[s] * a32n[s];
+                        info[s].relres = (rinn[s] + errb[s]) / bn[s];
+                    }
+            } else {
+                A.apply_into(x_.p, r_.p, p64, act_.p);
+                app64++;
+                launch_axpby(n, 1.0, b_.p, -1.0, r_.p, st);
+                seg_norms(r_.p, rn);
+                // the inner operator's error on this step's correction
+                launch_axpby(n, 1.0, r_.p, -1.0, rin_.p, st);
+                seg_norms(rin_.p, dn);
+                for (int s = 0; s < B; s++)
+                    if (a2[s]) {
+                        info[s].relres = rn[s] / bn[s];
+                        eps[s] = a32n[s] > 0.0 ? dn[s] / a32n[s] : -1.0;
+                        errb[s] = 0.0;
+                    }
+                if (verbose_) {
+                    double we = 0.0, wr = 0.0;
+                    for (int s = 0; s < B; s++)
+                        if (a2[s]) {
+                            we = std::max(we, eps[s]);
+                            wr = std::max(wr, info[s].relres);
+                        }
+                    std::fprintf(stderr, "[sphstk] refinement step %d: FP64 relres %.3e, measured inner-operator error %.3e\n",
+                                 outer + 1, wr, we);
                 }
+            }
+            for (int s = 0; s < B; s++)
+                if (a2[s]) info[s].iters += inner[s].iters;
             outer++;
         }
         for (int s = 0; s < B; s++)

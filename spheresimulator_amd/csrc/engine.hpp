@@ -108,14 +108,16 @@ struct FarLevel {
     DevBuf<double> src64;
     DevBuf<Chunk> chunks;
     DevBuf<double> d_down[kMaxOrder + 1], d_up[kMaxOrder + 1];
+    DevBuf<float> d_down32[kMaxOrder + 1], d_up32[kMaxOrder + 1]; // FP32 copies (FP32 level sets)
     ResampleTabs down, up;
     FarLists lists;
 };
 
 // Source-cell aggregation of the two coarsest FP32 levels (the treecode part
-// of the far field): cells of kCellSize Morton-consecutive spheres, each
+// of the far field): cells of kCellSize spheres (the leaves of a balanced k-d
+// split of each suspension's centres), each
 // represented beyond kCellTheta cell radii by the K equivalent point forces of
-// equivalent_tables(); a (target sphere, cell) pair whose members all lie in
+// equiv_tables(); a (target sphere, cell) pair whose members all lie in
 // levels 2-3 of the target interacts through them on the target's grid of the
 // finer of its members' levels.
 constexpr int kCellSize = 16;
@@ -358,8 +360,10 @@ class Mobility {
 
   private:
     // GMRES on the segments flagged in `act` (host, one per suspension); x and b device
+    // res (optional): the final residual b - A x of every suspension, from the
+    // last cycle's basis (no operator apply)
     void gmres(const double *b, double *x, bool x_zero, double tol, int prec, std::vector<int> &act,
-               std::vector<SegInfo> &info, int &applies);
+               std::vector<SegInfo> &info, int &applies, double *res = nullptr);
     void seg_norms(const double *v, std::vector<double> &out);
 
     std::unique_ptr<Operator> A_;
@@ -372,14 +376,14 @@ class Mobility {
     DevBuf<double> z_, zt_;      // preconditioned basis vector / update
     bool x_zero_ = true; // the stored unknown is known to be zero (cold start)
     bool verbose_ = false; // SPHSTK_VERBOSE=1: residual history on stderr (Belos Verbosity 63)
-    DevBuf<double> rho_, b_, x_, r_, d_, V_, h_, scal_, coefv_, force_, vel_, gmres_inner_rhs_, mdot_;
+    DevBuf<double> rho_, b_, x_, r_, d_, V_, h_, scal_, coefv_, force_, vel_, gmres_inner_rhs_, mdot_, rin_;
+    bool explicit_refine_ = false; // mixed form: an explicit FP64 residual at every refinement step
     DevBuf<int> act_;
     // device-side GMRES least-squares state (per suspension) and the pinned
     // count of active suspensions read one iteration late behind an event
     DevBuf<double> d_H_, d_cs_, d_sn_, d_g_, d_bn_, d_beta_, d_relres_;
     DevBuf<int> d_cyc_, d_kdim_, d_impok_, d_nact_;
     int *nact_host_ = nullptr;
-    int last_kmax_ = 0, last_m_ = 0, last_prec_ = -1; // the last GMRES cycle's basis size, restart and operator
     hipEvent_t nact_ev_[2] = {nullptr, nullptr};
 };
 

@@ -438,108 +438,121 @@ void launch_equiv_forces(int ncell, int K, int Kc, const double *P, const double
 
 // ---------------------------------------------------------------------------
 // Per-sphere matrices applied to body-frame values, one tile of up to kTileSph
-// spheres of the same order per workgroup so that each matrix element is read
-// once per tile (the matrices are up to 12.7 MB at p = 16):
+// = 16 spheres of the same order per workgroup, as a GEMM on the matrix cores
+// (the tables are up to 12.7 MB at p = 16; each element is read once per tile):
 //   analysis:  coef[S] = A_p (q_S^-1 in_S)                (A_p: 6Q x 3n)
 //   resample:  out_S (+)= q_S M_p (q_S^-1 in_S)            (M_p: 3 nout x 3 nin)
-// A workgroup owns kRowBlock = 63 matrix rows (21 points) of one tile: lane l
-// of each of its four waves owns row r0 + l and the wave sums a quarter of
-// the columns of every chunk; the four partial sums meet in LDS at the end.
-// The rotated input values are staged through LDS in chunks of kChunkPts
-// points, stored [column][sphere] so the inner loop reads them as broadcasts.
-constexpr int kRowBlock = 63;
+// A workgroup owns kRowBlock = 192 matrix rows (64 points): each of its four
+// waves three blocks of 16 rows, accumulated by v_mfma_f64_16x16x4f64 over the
+// columns (16 rows x 4 columns of the matrix, read from global memory, times
+// 4 columns x 16 spheres of the rotated input).  The input is staged through
+// LDS in chunks of kChunkPts points, stored [column][sphere].  Operand layout
+// of the instruction: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
+// D[i][j] in lane j + 16 (i % 4), element i / 4.
+// The FP32 inner operator's level resampling runs the same GEMM with FP32
+// tables, FP32 staging and v_mfma_f32_16x16x4f32 (same A and B layout, D[i][j]
+// in lane j + 16 (i / 4), element i % 4; FP32 accumulation: about 1e-7
+// relative against the FP64 resampling, far below that operator's own 4e-6).
+constexpr int kRowBlock = 192;
+constexpr int kWaveRowBlocks = 3;
 constexpr int kChunkPts = 64;
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 
-struct MatTabs {
-    const double *M[kMaxOrder + 1] = {};
+template <typename T>
+struct MatTabsT {
+    const T *M[kMaxOrder + 1] = {};
     int rows[kMaxOrder + 1] = {};
     int nin[kMaxOrder + 1] = {};
 };
+using MatTabs = MatTabsT<double>;
 
-template <bool RESAMPLE>
-__global__ __launch_bounds__(256) void k_tiled_apply(DevGeom g, MatTabs tab, const int64_t *__restrict__ in_off,
+__device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ flt4 mfma16x16x4(float a, float b, flt4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <bool RESAMPLE, typename T>
+__global__ __launch_bounds__(256) void k_tiled_apply(DevGeom g, MatTabsT<T> tab, const int64_t *__restrict__ in_off,
                                                      const int64_t *__restrict__ out_off,
                                                      const double *__restrict__ in, double *__restrict__ out,
                                                      int accumulate, const int *__restrict__ active) {
-    __shared__ double V[3 * kChunkPts * kTileSph];
-    __shared__ double P[4][64 * kTileSph];
-    const SphTile T = g.tiles[blockIdx.x];
-    const int p = T.p, rows = tab.rows[p], nin = tab.nin[p];
+    typedef T T4 __attribute__((ext_vector_type(4)));
+    __shared__ T V[3 * kChunkPts * kTileSph];
+    __shared__ T P[kRowBlock * kTileSph];
+    const SphTile Ti = g.tiles[blockIdx.x];
+    const int p = Ti.p, rows = tab.rows[p], nin = tab.nin[p];
     const int r0 = blockIdx.y * kRowBlock;
     if (r0 >= rows) return;
-    const double *__restrict__ M = tab.M[p];
-    const int *sph = g.tile_sph + T.first;
+    const T *__restrict__ M = tab.M[p];
+    const int *sph = g.tile_sph + Ti.first;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int r = r0 + lane;
-    const bool own = lane < kRowBlock && r < rows;
-    double acc[kTileSph];
+    const int li = lane & 15, kq = lane >> 4;
+    const int rw = r0 + wv * 16 * kWaveRowBlocks; // this wave's first row
+    T4 acc[kWaveRowBlocks];
 #pragma unroll
-    for (int t = 0; t < kTileSph; t++) acc[t] = 0.0;
+    for (int rb = 0; rb < kWaveRowBlocks; rb++) acc[rb] = T4{T(0), T(0), T(0), T(0)};
     for (int k0 = 0; k0 < nin; k0 += kChunkPts) {
         const int kc = min(kChunkPts, nin - k0);
         __syncthreads();
         for (int e = tid; e < kChunkPts * kTileSph; e += blockDim.x) {
             const int t = e / kChunkPts, i = e % kChunkPts;
             double o[3] = {0.0, 0.0, 0.0};
-            if (t < T.count && i < kc) {
+            if (t < Ti.count && i < kc) {
                 const int S = sph[t];
                 const double *v = in + 3 * (in_off[S] + k0 + i);
                 const double vv[3] = {v[0], v[1], v[2]};
                 qrot(g.quat + 4 * S, vv, o, true);
             }
-            V[(3 * i) * kTileSph + t] = o[0];
-            V[(3 * i + 1) * kTileSph + t] = o[1];
-            V[(3 * i + 2) * kTileSph + t] = o[2];
+            V[(3 * i) * kTileSph + t] = (T)o[0];
+            V[(3 * i + 1) * kTileSph + t] = (T)o[1];
+            V[(3 * i + 2) * kTileSph + t] = (T)o[2];
         }
         __syncthreads();
-        if (own) {
-            // this wave's quarter of the chunk's columns, four matrix loads in flight
-            const int nc = 3 * kc, cq = (nc + 3) / 4, c0 = wv * cq, c1 = min(nc, c0 + cq);
-            const double *Mc = M + (int64_t)(3 * k0) * rows + r;
-            int c = c0;
-            for (; c + 4 <= c1; c += 4) {
-                double a[4];
+        const int nc = 3 * kc;
+        const T *Mc = M + (int64_t)(3 * k0) * rows;
+        for (int c = 0; c < nc; c += 4) {
+            const int col = c + kq;
+            const bool cin = col < nc;
+            const T b = cin ? V[col * kTileSph + li] : T(0);
 #pragma unroll
-                for (int u = 0; u < 4; u++) a[u] = Mc[(int64_t)(c + u) * rows];
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const double *vc = V + (c + u) * kTileSph;
-#pragma unroll
-                    for (int t = 0; t < kTileSph; t++) acc[t] = __builtin_fma(a[u], vc[t], acc[t]);
-                }
-            }
-            for (; c < c1; c++) {
-                const double a = Mc[(int64_t)c * rows];
-                const double *vc = V + c * kTileSph;
-#pragma unroll
-                for (int t = 0; t < kTileSph; t++) acc[t] = __builtin_fma(a, vc[t], acc[t]);
+            for (int rb = 0; rb < kWaveRowBlocks; rb++) {
+                const int r = rw + 16 * rb + li;
+                const T a = (cin && r < rows) ? Mc[(int64_t)col * rows + r] : T(0);
+                acc[rb] = mfma16x16x4(a, b, acc[rb]);
             }
         }
     }
-    // the four waves' partial sums, added in a fixed order
+    // row of element q of this lane's accumulator within its 16-row block:
+    // FP32 instruction 4 (lane / 16) + q, FP64 instruction (lane / 16) + 4 q
 #pragma unroll
-    for (int t = 0; t < kTileSph; t++) P[wv][lane * kTileSph + t] = acc[t];
-    __syncthreads();
-    for (int e = tid; e < 64 * kTileSph; e += blockDim.x) P[0][e] = (P[0][e] + P[1][e]) + (P[2][e] + P[3][e]);
+    for (int rb = 0; rb < kWaveRowBlocks; rb++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int i = sizeof(T) == 8 ? kq + 4 * q : 4 * kq + q;
+            P[(wv * 16 * kWaveRowBlocks + 16 * rb + i) * kTileSph + li] = acc[rb][q];
+        }
     __syncthreads();
     if (!RESAMPLE) {
         const int nr = min(rows, r0 + kRowBlock) - r0;
-        for (int e = tid; e < nr * T.count; e += blockDim.x) {
+        for (int e = tid; e < nr * Ti.count; e += blockDim.x) {
             const int t = e / nr, i = e % nr;
             const int S = sph[t];
             if (active && !active[g.sph_susp[S]]) continue;
-            out[g.coef_off[S] + r0 + i] = P[0][i * kTileSph + t];
+            out[g.coef_off[S] + r0 + i] = P[i * kTileSph + t];
         }
         return;
     }
     // resample: rotate each output point back to the lab frame
     const int npt = (min(rows, r0 + kRowBlock) - r0) / 3;
-    for (int e = tid; e < npt * T.count; e += blockDim.x) {
+    for (int e = tid; e < npt * Ti.count; e += blockDim.x) {
         const int t = e / npt, i = e % npt;
         const int S = sph[t];
         if (active && !active[g.sph_susp[S]]) continue;
-        const double vb[3] = {P[0][(3 * i) * kTileSph + t], P[0][(3 * i + 1) * kTileSph + t],
-                              P[0][(3 * i + 2) * kTileSph + t]};
+        const double vb[3] = {(double)P[(3 * i) * kTileSph + t], (double)P[(3 * i + 1) * kTileSph + t],
+                              (double)P[(3 * i + 2) * kTileSph + t]};
         double o[3];
         qrot(g.quat + 4 * S, vb, o, false);
         double *dst = out + 3 * (out_off[S] + r0 / 3 + i);
@@ -559,14 +572,26 @@ void launch_resample(const DevGeom &g, const ResampleTabs &tab, const int64_t *i
                      const double *in, double *out, int accumulate, const int *active, hipStream_t s,
                      int max_points) {
     if (!g.nsph || !g.ntiles) return;
+    const int rb = (3 * max_points + kRowBlock - 1) / kRowBlock;
+    if (tab.fp32) {
+        MatTabsT<float> mt;
+        for (int p = 0; p <= kMaxOrder; p++) {
+            mt.M[p] = tab.M32[p];
+            mt.rows[p] = 3 * tab.nout[p];
+            mt.nin[p] = tab.nin[p];
+        }
+        k_tiled_apply<true, float><<<dim3(g.ntiles, rb), dim3(256), 0, s>>>(g, mt, in_off, out_off, in, out,
+                                                                           accumulate, active);
+        return;
+    }
     MatTabs mt;
     for (int p = 0; p <= kMaxOrder; p++) {
         mt.M[p] = tab.M[p];
         mt.rows[p] = 3 * tab.nout[p];
         mt.nin[p] = tab.nin[p];
     }
-    const int rb = (3 * max_points + kRowBlock - 1) / kRowBlock;
-    k_tiled_apply<true><<<dim3(g.ntiles, rb), dim3(256), 0, s>>>(g, mt, in_off, out_off, in, out, accumulate, active);
+    k_tiled_apply<true, double><<<dim3(g.ntiles, rb), dim3(256), 0, s>>>(g, mt, in_off, out_off, in, out, accumulate,
+                                                                        active);
 }
 void launch_analysis(const DevGeom &g, const double *in, double *coef, const int *active, hipStream_t s,
                      int max_order) {
@@ -578,7 +603,8 @@ void launch_analysis(const DevGeom &g, const double *in, double *coef, const int
         mt.nin[p] = (p + 1) * (2 * p + 2);
     }
     const int rb = (3 * (max_order + 1) * (max_order + 2) + kRowBlock - 1) / kRowBlock;
-    k_tiled_apply<false><<<dim3(g.ntiles, rb), dim3(256), 0, s>>>(g, mt, g.pt_off, nullptr, in, coef, 0, active);
+    k_tiled_apply<false, double><<<dim3(g.ntiles, rb), dim3(256), 0, s>>>(g, mt, g.pt_off, nullptr, in, coef, 0,
+                                                                         active);
 }
 
 // ---------------------------------------------------------------------------
@@ -1060,6 +1086,11 @@ __global__ __launch_bounds__(256) void k_seg_scale(double *__restrict__ y, const
     for (int64_t i = c.begin + threadIdx.x; i < c.end; i += blockDim.x) y[i] = a * x[i];
 }
 
+// x (+)= sum_j coef[j nseg + seg] V_j over the active segments (SET: x = the
+// sum).  Zero coefficients skip their vector: a suspension that left a GMRES
+// cycle early has basis vectors beyond its dimension that this cycle never
+// wrote.
+template <bool SET>
 __global__ __launch_bounds__(256) void k_seg_multi_axpy(double *__restrict__ x, const double *__restrict__ V,
                                                         int64_t ldv, const double *__restrict__ coef, int k,
                                                         int nseg, const Chunk *__restrict__ ch,
@@ -1067,8 +1098,11 @@ __global__ __launch_bounds__(256) void k_seg_multi_axpy(double *__restrict__ x, 
     const Chunk c = ch[blockIdx.x];
     if (active && !active[c.seg]) return;
     for (int64_t i = c.begin + threadIdx.x; i < c.end; i += blockDim.x) {
-        double s = x[i];
-        for (int j = 0; j < k; j++) s = __builtin_fma(coef[(int64_t)j * nseg + c.seg], V[(int64_t)j * ldv + i], s);
+        double s = SET ? 0.0 : x[i];
+        for (int j = 0; j < k; j++) {
+            const double cf = coef[(int64_t)j * nseg + c.seg];
+            if (cf != 0.0) s = __builtin_fma(cf, V[(int64_t)j * ldv + i], s);
+        }
         x[i] = s;
     }
 }
@@ -1209,44 +1243,32 @@ __global__ void k_gmres_backsub(int B, int m, const double *__restrict__ H, cons
     }
 }
 
-// k_gmres_project: y_s = argmin |c_s - Hbar_s y| through the QR of the last
-// cycle's Hessenberg matrix (its Givens rotations applied to c_s = V^T r, then
-// back substitution with R), as coef[i B + s]; zero for suspensions not in
-// act or with an empty space.  With A V_k = V_{k+1} Hbar this is the
-// minimum-residual correction of r within span(V_k).
-__global__ void k_gmres_project(int B, int m, const double *__restrict__ c, const double *__restrict__ H,
-                                const double *__restrict__ cs, const double *__restrict__ sn,
-                                const int *__restrict__ kdim, const int *__restrict__ act, double *__restrict__ coef) {
+// k_gmres_resid_coef: coefficients of the residual at the end of a cycle in
+// the cycle's basis.  With Omega = G_{k-1} ... G_0 the product of the Givens
+// rotations (Omega Hbar = [R; 0], Omega beta e_1 = g) and y = R^-1 g[0..k),
+//   r = V_{k+1} (beta e_1 - Hbar y) = V_{k+1} Omega^T (0, ..., 0, g_k)^T,
+// so coef = g_k G_0^T ... G_{k-1}^T e_k (k + 1 entries, as coef[i B + s]);
+// zero for suspensions outside the cycle.
+__global__ void k_gmres_resid_coef(int B, int m, const double *__restrict__ g, const double *__restrict__ cs,
+                                   const double *__restrict__ sn, const int *__restrict__ kdim,
+                                   const int *__restrict__ cyc, double *__restrict__ coef) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= B) return;
-    for (int i = 0; i < m; i++) coef[(size_t)i * B + s] = 0.0;
+    for (int i = 0; i <= m; i++) coef[(size_t)i * B + s] = 0.0;
     const int k = kdim[s];
-    if (!act[s] || k <= 0) return;
-    const double *Hs = H + (size_t)s * (m + 1) * m;
-    const double *cv = cs + (size_t)s * m, *sv = sn + (size_t)s * m;
-    // rotated c, kept in coef's column for s (k + 1 <= m + 1 entries: the last in a register)
-    double last = c[(size_t)k * B + s];
-    for (int i = 0; i <= k - 1; i++) coef[(size_t)i * B + s] = c[(size_t)i * B + s];
-    for (int i = 0; i < k; i++) {
-        const double ti = coef[(size_t)i * B + s];
-        const double tn = (i + 1 < k) ? coef[(size_t)(i + 1) * B + s] : last;
-        coef[(size_t)i * B + s] = cv[i] * ti + sv[i] * tn;
-        const double t1 = -sv[i] * ti + cv[i] * tn;
-        if (i + 1 < k)
-            coef[(size_t)(i + 1) * B + s] = t1;
-        else
-            last = t1;
-    }
-    for (int i = k - 1; i >= 0; i--) {
-        double acc = coef[(size_t)i * B + s];
-        for (int q = i + 1; q < k; q++) acc -= Hs[i * m + q] * coef[(size_t)q * B + s];
-        coef[(size_t)i * B + s] = Hs[i * m + i] != 0.0 ? acc / Hs[i * m + i] : 0.0;
+    if (!cyc[s] || k <= 0) return;
+    const double *c = cs + (size_t)s * m, *sv = sn + (size_t)s * m;
+    coef[(size_t)k * B + s] = g[(size_t)s * (m + 1) + k];
+    for (int j = k - 1; j >= 0; j--) {
+        const double a = coef[(size_t)j * B + s], b = coef[(size_t)(j + 1) * B + s];
+        coef[(size_t)j * B + s] = c[j] * a - sv[j] * b;
+        coef[(size_t)(j + 1) * B + s] = sv[j] * a + c[j] * b;
     }
 }
 
-void launch_gmres_project(int B, int m, const double *c, const double *H, const double *cs, const double *sn,
-                          const int *kdim, const int *act, double *coef, hipStream_t s) {
-    if (B > 0) k_gmres_project<<<dim3((B + 255) / 256), dim3(256), 0, s>>>(B, m, c, H, cs, sn, kdim, act, coef);
+void launch_gmres_resid_coef(int B, int m, const double *g, const double *cs, const double *sn, const int *kdim,
+                             const int *cyc, double *coef, hipStream_t s) {
+    if (B > 0) k_gmres_resid_coef<<<dim3((B + 255) / 256), dim3(256), 0, s>>>(B, m, g, cs, sn, kdim, cyc, coef);
 }
 
 void launch_gmres_init(int B, int m, const double *beta, const int *cyc, double *g, int *cur, int *kdim, int *impok,
@@ -1311,7 +1333,12 @@ void launch_seg_scale(double *y, const double *x, const double *alpha, const Chu
 void launch_seg_multi_axpy(double *x, const double *V, int64_t ldv, const double *coef, int k, int nseg,
                            const Chunk *chunks, int nchunks, const int *active, hipStream_t s) {
     if (nchunks > 0 && k > 0)
-        k_seg_multi_axpy<<<dim3(nchunks), dim3(256), 0, s>>>(x, V, ldv, coef, k, nseg, chunks, active);
+        k_seg_multi_axpy<false><<<dim3(nchunks), dim3(256), 0, s>>>(x, V, ldv, coef, k, nseg, chunks, active);
+}
+void launch_seg_multi_comb(double *x, const double *V, int64_t ldv, const double *coef, int k, int nseg,
+                           const Chunk *chunks, int nchunks, const int *active, hipStream_t s) {
+    if (nchunks > 0 && k > 0)
+        k_seg_multi_axpy<true><<<dim3(nchunks), dim3(256), 0, s>>>(x, V, ldv, coef, k, nseg, chunks, active);
 }
 void launch_seg_multi_dot(const double *w, const double *V, int64_t ldv, int k, const Chunk *chunks, int nchunks,
                           double *partial, const int *active, hipStream_t s) {

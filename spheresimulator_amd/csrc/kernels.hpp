@@ -44,8 +44,13 @@ struct Chunk {
 };
 
 // Per-order resampling matrices on device (see Resample in common.hpp).
+// per-order resampling matrices (column-major, 3 nout x 3 nin); M32: the same
+// in FP32 (set for the FP32 inner operator's levels, whose resampling then runs
+// on FP32 matrix cores with FP32 accumulation), else NULL
 struct ResampleTabs {
+    bool fp32 = false;
     const double *M[kMaxOrder + 1] = {};
+    const float *M32[kMaxOrder + 1] = {};
     int nin[kMaxOrder + 1] = {};
     int nout[kMaxOrder + 1] = {};
 };
@@ -131,12 +136,18 @@ void launch_seg_multi_reduce(const double *partial, int nchunks, const int *off,
                              const int *active, hipStream_t s);
 void launch_seg_multi_axpy(double *x, const double *V, int64_t ldv, const double *coef, int k, int nseg,
                            const Chunk *chunks, int nchunks, const int *active, hipStream_t s);
+// x = sum_j coef[j nseg + seg] V_j over the active segments (x overwritten there)
+void launch_seg_multi_comb(double *x, const double *V, int64_t ldv, const double *coef, int k, int nseg,
+                           const Chunk *chunks, int nchunks, const int *active, hipStream_t s);
 // GMRES least-squares bookkeeping on the device, one Krylov space per
 // suspension (arrays: H B x (m+1) x m, cs/sn B x m, g B x (m+1), per-suspension
 // ints and doubles of length B): cycle start (g = beta e_1, cur = cyc), the
 // Givens update of column j from the orthogonalisation products hh (active set
 // cur shrinks as suspensions converge; *nact = those still active), and the
 // back substitution into coef[i B + s].
+// residual coefficients of a finished cycle in its basis (m + 1 per suspension, coef[i B + s])
+void launch_gmres_resid_coef(int B, int m, const double *g, const double *cs, const double *sn, const int *kdim,
+                             const int *cyc, double *coef, hipStream_t s);
 void launch_gmres_init(int B, int m, const double *beta, const int *cyc, double *g, int *cur, int *kdim, int *impok,
                        hipStream_t s);
 void launch_gmres_givens(int B, int j, int m, const double *hh, double hsign, double *H, double *cs, double *sn,
@@ -144,11 +155,6 @@ void launch_gmres_givens(int B, int j, int m, const double *hh, double hsign, do
                          int *nact, hipStream_t s);
 void launch_gmres_backsub(int B, int m, const double *H, const double *g, const int *kdim, const int *cyc,
                           double *coef, hipStream_t s);
-// the minimum-residual coefficients of r within the last cycle's Krylov space
-// span(V_k) from c = V_{k+1}^T r (c[i B + s], i <= kdim_s) and that cycle's
-// rotated Hessenberg matrix and rotations, as coef[i B + s]
-void launch_gmres_project(int B, int m, const double *c, const double *H, const double *cs, const double *sn,
-                          const int *kdim, const int *act, double *coef, hipStream_t s);
 // out += sum of nslot partial-sum slots (far-field source slices)
 void launch_slot_reduce(double *out, const double *part, int nslot, int64_t stride, const Chunk *chunks,
                         int nchunks, const int *active, hipStream_t s);

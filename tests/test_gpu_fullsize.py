@@ -140,3 +140,33 @@ def test_fullsize_mixed_solve_against_fp64_and_oracle(cfg2, fp64_solve):
     res = float(np.linalg.norm(r) / np.linalg.norm(b[rows]))
     print("oracle residual of the mixed density on sampled spheres %.3e" % res)
     assert res < 1e-7
+
+
+def test_fullsize_mixed_residual_update_against_explicit_steps(cfg2):
+    """Refinement steps after the first take the inner solve's own residual
+    plus the measured bound on the inner operator's error in place of an FP64
+    apply; SPHSTK_FLAG_EXPLICIT_REFINE takes an FP64 residual at every step.
+    The default makes fewer FP64 applies, returns the same velocities, and the
+    FP64 direct operator's residual of its density is below the tolerance."""
+    c = cfg2["c"]
+    say("1024 spheres p=12: mixed solve, residual updates vs explicit FP64 residuals")
+    out = {}
+    for explicit in (False, True):
+        M = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], precision=S.PREC_MIXED,
+                              explicit_refine=explicit)
+        v, info = M.solveMob(c["force"])
+        print("explicit=%d: outer %d inner %d FP64 applies %d relres %.3e" % (
+            explicit, info["outer"], info["iters"], info["applies_fp64"], info["relres"]))
+        assert info["converged"] == 1 and info["relres"] <= 1e-8
+        out[explicit] = (v, info, M.get_unknown())
+        del M
+    assert out[False][1]["applies_fp64"] < out[True][1]["applies_fp64"]
+    a, b = out[False][0].reshape(-1, 6), out[True][0].reshape(-1, 6)
+    eV, eW = rel(a[:, :3], b[:, :3]), rel(a[:, 3:], b[:, 3:])
+    print("updated vs explicit residuals: rel diff V %.3e Omega %.3e" % (eV, eW))
+    assert eV < 1e-8 and eW < 1e-8
+    op = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+    rhs = op.applyP2POP(rho0_of(cfg2), np.zeros(op.n_dof), -0.5, 0.0, -1.0)
+    res = float(np.linalg.norm(rhs - op.apply(out[False][2])) / np.linalg.norm(rhs))
+    print("FP64 direct residual of the density: %.3e (reported bound %.3e)" % (res, out[False][1]["relres"]))
+    assert res <= 1e-8
