@@ -2090,10 +2090,17 @@ void Mobility::solve(const double *force, double *vel, std::vector<SegInfo> &inf
             launch_axpby(n, -1.0, rin_.p, 1.0, gmres_inner_rhs_.p, st);
             seg_norms(rin_.p, rinn);
             seg_norms(gmres_inner_rhs_.p, a32n);
-            bool update = !explicit_refine_;
-            for (int s = 0; s < B; s++)
-                if (a2[s] && !(eps[s] >= 0.0 && errb[s] + 2.0 * eps[s] * a32n[s] <= 0.5 * prm_.tol * bn[s]))
-                    update = false;
+            // per suspension: the updated residual where its bound allows,
+            // else an FP64 apply (masked to those suspensions)
+            std::vector<int> upd(B, 0), exp_(B, 0);
+            int nupd = 0, nexp = 0;
+            for (int s = 0; s < B; s++) {
+                if (!a2[s]) continue;
+                const bool ok = !explicit_refine_ && eps[s] >= 0.0 &&
+                                errb[s] + 2.0 * eps[s] * a32n[s] <= 0.5 * prm_.tol * bn[s];
+                (ok ? upd : exp_)[s] = 1;
+                (ok ? nupd : nexp)++;
+            }
             if (verbose_) {
                 double we = -1.0, wb = 0.0;
                 for (int s = 0; s < B; s++)
@@ -2101,17 +2108,12 @@ void Mobility::solve(const double *force, double *vel, std::vector<SegInfo> &inf
                         we = std::max(we, eps[s]);
                         wb = std::max(wb, (errb[s] + 2.0 * eps[s] * a32n[s]) / bn[s]);
                     }
-                std::fprintf(stderr, "[sphstk] refinement step %d: max inner-operator error %.3e, residual bound %.3e: %s\n",
-                             outer + 1, we, wb, update ? "updated residual" : "FP64 residual");
+                std::fprintf(stderr, "[sphstk] refinement step %d: max inner-operator error %.3e, residual bound %.3e: "
+                             "%d updated residuals, %d FP64 residuals\n", outer + 1, we, wb, nupd, nexp);
             }
-            if (update) {
-                launch_axpby(n, 1.0, rin_.p, 0.0, r_.p, st);
-                for (int s = 0; s < B; s++)
-                    if (a2[s]) {
-                        errb[s] += 2.0 * eps[s] * a32n[s];
-                        info[s].relres = (rinn[s] + errb[s]) / bn[s];
-                    }
-            } else {
+            if (nupd && nexp) launch_axpby(n, 1.0, rin_.p, 0.0, d_.p, st); // the correction is in x already
+            if (nexp) {
+                SPHSTK_HIP(hipMemcpyAsync(act_.p, exp_.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
                 A.apply_into(x_.p, r_.p, p64, act_.p);
                 app64++;
                 launch_axpby(n, 1.0, b_.p, -1.0, r_.p, st);
@@ -2120,7 +2122,7 @@ void Mobility::solve(const double *force, double *vel, std::vector<SegInfo> &inf
                 launch_axpby(n, 1.0, r_.p, -1.0, rin_.p, st);
                 seg_norms(rin_.p, dn);
                 for (int s = 0; s < B; s++)
-                    if (a2[s]) {
+                    if (exp_[s]) {
                         info[s].relres = rn[s] / bn[s];
                         eps[s] = a32n[s] > 0.0 ? dn[s] / a32n[s] : -1.0;
                         errb[s] = 0.0;
@@ -2128,13 +2130,28 @@ void Mobility::solve(const double *force, double *vel, std::vector<SegInfo> &inf
                 if (verbose_) {
                     double we = 0.0, wr = 0.0;
                     for (int s = 0; s < B; s++)
-                        if (a2[s]) {
+                        if (exp_[s]) {
                             we = std::max(we, eps[s]);
                             wr = std::max(wr, info[s].relres);
                         }
                     std::fprintf(stderr, "[sphstk] refinement step %d: FP64 relres %.3e, measured inner-operator error %.3e\n",
                                  outer + 1, wr, we);
                 }
+            }
+            if (nupd) {
+                const double *src = nexp ? d_.p : rin_.p;
+                if (nexp) {
+                    // r = rin on the updated suspensions only (ones at scal_ + 2B)
+                    SPHSTK_HIP(hipMemcpyAsync(act_.p, upd.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
+                    launch_seg_scale(r_.p, src, scal_.p + 2 * B, A.chunks(), A.nchunks(), act_.p, st);
+                } else {
+                    launch_axpby(n, 1.0, src, 0.0, r_.p, st);
+                }
+                for (int s = 0; s < B; s++)
+                    if (upd[s]) {
+                        errb[s] += 2.0 * eps[s] * a32n[s];
+                        info[s].relres = (rinn[s] + errb[s]) / bn[s];
+                    }
             }
             for (int s = 0; s < B; s++)
                 if (a2[s]) info[s].iters += inner[s].iters;
