@@ -19,17 +19,26 @@ _M2 = np.uint64(0x94D049BB133111EB)
 class Stream:
     """Counter-based SplitMix64: value i = mix(seed + (i + 1) * gamma)."""
 
+    _BLOCK = 65536
+
     def __init__(self, seed):
         self.seed = np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
         self.ctr = 0
+        self._buf = np.zeros(0, dtype=np.uint64)  # values ctr .. ctr + len - 1, computed ahead
 
-    def u64(self, n):
+    def _mix(self, first, n):
         with np.errstate(over="ignore"):
-            i = np.arange(self.ctr + 1, self.ctr + 1 + n, dtype=np.uint64)
+            i = np.arange(first + 1, first + 1 + n, dtype=np.uint64)
             z = self.seed + i * _GAMMA
             z = (z ^ (z >> np.uint64(30))) * _M1
             z = (z ^ (z >> np.uint64(27))) * _M2
-            z = z ^ (z >> np.uint64(31))
+            return z ^ (z >> np.uint64(31))
+
+    def u64(self, n):
+        if n > len(self._buf):
+            self._buf = self._mix(self.ctr, max(n, self._BLOCK))
+        z = self._buf[:n]
+        self._buf = self._buf[n:]
         self.ctr += n
         return z
 
@@ -62,33 +71,44 @@ class Stream:
 class _Grid:
     """Cell hash for non-overlap checks during sequential placement."""
 
+    _OFF = [(dx, dy, dz) for dx in (-1, 0, 1) for dy in (-1, 0, 1) for dz in (-1, 0, 1)]
+
     def __init__(self, h):
         self.h = h
         self.cells = {}
-        self.pos = []
-        self.rad = []
+        self.n = 0
+        self._p = np.zeros((1024, 3))
+        self._r = np.zeros(1024)
+
+    @property
+    def pos(self):
+        return self._p[:self.n]
+
+    @property
+    def rad(self):
+        return self._r[:self.n]
 
     def key(self, x):
         return (int(math.floor(x[0] / self.h)), int(math.floor(x[1] / self.h)), int(math.floor(x[2] / self.h)))
 
     def ok(self, x, r, gap_fn, skip=-1):
-        k = self.key(x)
-        for dx in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dz in (-1, 0, 1):
-                    for j in self.cells.get((k[0] + dx, k[1] + dy, k[2] + dz), ()):
-                        if j == skip:
-                            continue
-                        d = x - self.pos[j]
-                        lim = r + self.rad[j] + gap_fn(r, self.rad[j])
-                        if d @ d < lim * lim:
-                            return False
-        return True
+        k0, k1, k2 = self.key(x)
+        get = self.cells.get
+        idx = [j for dx, dy, dz in self._OFF for j in get((k0 + dx, k1 + dy, k2 + dz), ()) if j != skip]
+        if not idx:
+            return True
+        d = x - self._p[idx]
+        lim = r + self._r[idx] + gap_fn(r, self._r[idx])
+        return not bool(np.any(np.einsum("ij,ij->i", d, d) < lim * lim))
 
     def add(self, x, r):
-        self.cells.setdefault(self.key(x), []).append(len(self.pos))
-        self.pos.append(np.asarray(x, dtype=np.float64))
-        self.rad.append(float(r))
+        if self.n == len(self._r):
+            self._p = np.concatenate([self._p, np.zeros_like(self._p)])
+            self._r = np.concatenate([self._r, np.zeros_like(self._r)])
+        self.cells.setdefault(self.key(x), []).append(self.n)
+        self._p[self.n] = x
+        self._r[self.n] = r
+        self.n += 1
 
 
 def _rsa(st, radii, lo, hi, gap_fn, max_tries=2000000, wall=True):
@@ -115,7 +135,7 @@ def _rsa(st, radii, lo, hi, gap_fn, max_tries=2000000, wall=True):
             if g.ok(x, r, gap_fn):
                 g.add(x, r)
                 break
-    return np.array(g.pos)
+    return g.pos.copy()
 
 
 def make_suspension(n, p, seed, vf=None, box=None, radius=1.0, min_gap=0.2, torque_std=1.0,
@@ -192,7 +212,7 @@ def config5(seed=5, n=16384, p=12, vf=0.2, chain=8):
     st = Stream(seed)
     radii = np.exp(0.2 * st.normal(n))
     box = (np.sum(4.0 / 3.0 * math.pi * radii ** 3) / vf) ** (1.0 / 3.0)
-    gap_fn = lambda a, b: 0.01 * min(a, b)  # noqa: E731
+    gap_fn = lambda a, b: 0.01 * np.minimum(a, b)  # noqa: E731
     g = _Grid(2 * radii.max() * 1.6)
     bonds = 0
     i = 0
