@@ -127,7 +127,7 @@ int sphstk_operator_apply_device(sphstk_operator *op, const double *dX, double *
  * (one column).  No counterpart in the reference: A32 has the far field in FP32
  * and, unless SPHSTK_FLAG_FP32_DIRECT was given at creation, multi-level (sphere
  * pairs farther apart than 3 (R_s + R_t) interact through coarser grids of order
- * p/2+1, max(5, p/3+1), 4 and 3 from 3, 6, 11 and 18 (R_s + R_t) on; orders
+ * p/2+1, max(5, p/3+1), 4 and 3 from 3, 6, 9 and 16 (R_s + R_t) on; orders
  * below 10 keep their own grid at the first level); self and near terms stay FP64;
  * relative error about 4e-6 against A64. */
 int sphstk_operator_apply_inner(sphstk_operator *op, const double *X, double *Y, double alpha, double beta);

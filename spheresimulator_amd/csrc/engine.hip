@@ -254,7 +254,7 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     int tpl = 1, split = 0;
     // FP32 inner operator: orders from level_order(); FP64 outer applies: fine
     // below 7 (R_s + R_t), then orders 8, 6, 5, 4 from 7, 14, 28, 56 (R_s + R_t).
-    ls32_.alo[0] = 3.0, ls32_.alo[1] = 6.0, ls32_.alo[2] = 11.0, ls32_.alo[3] = 18.0;
+    ls32_.alo[0] = 3.0, ls32_.alo[1] = 6.0, ls32_.alo[2] = 9.0, ls32_.alo[3] = 16.0;
     ls32_.order[0] = ls32_.order[1] = ls32_.order[2] = ls32_.order[3] = -1;
     ls32_.fp64 = false;
     ls32_.cells = (flags & 32) == 0;

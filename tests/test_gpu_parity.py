@@ -272,11 +272,14 @@ def test_inner_fp32_operator_accuracy(p, direct, tol):
 
 def test_inner_operator_source_cells():
     """Source-cell aggregation of the FP32 inner operator's two coarsest levels
-    (cells of 16 Morton-consecutive spheres as 200 equivalent point forces,
-    used for targets beyond 4 cell radii): on a dense suspension where cells
-    are in use it evaluates fewer pairs, and the operator stays as close to the
-    FP64 direct-sum operator as without cells (band-limited and white
-    densities), and to the oracle's direct sums on sampled spheres."""
+    (cells of 16 spheres, the leaves of a k-d split, as 200 equivalent point
+    forces, used for targets beyond 4 cell radii): on a dense suspension where
+    cells are in use it evaluates fewer pairs, and the operator stays as close
+    to the FP64 direct-sum operator as without cells (band-limited and white
+    densities), and to the oracle's direct sums on sampled spheres.  The
+    operator's own error at this density (p=8, vf 0.2) is about 1.2e-5 with
+    the default levels; the refinement takes its cheap second step for
+    errors up to about 5e-5."""
     say("inner FP32 operator with source cells (4096 spheres p=8 vf 0.2)")
     c = S.configs.make_suspension(4096, 8, 37, vf=0.2, min_gap=0.1)
     args = (c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
@@ -288,7 +291,7 @@ def test_inner_operator_source_cells():
         y64 = with_cells.apply(x)
         e_cells, e_plain = rel(with_cells.apply_inner(x), y64), rel(without.apply_inner(x), y64)
         print("%s: inner rel err with cells %.3e, without %.3e" % (name, e_cells, e_plain))
-        assert e_cells < 1e-5 and e_cells < 1.5 * e_plain + 2e-6
+        assert e_cells < 2e-5 and e_cells < 1.5 * e_plain + 2e-6
     p_cells, p_plain = with_cells.kernel_stats(0, 1)["pairs"], without.kernel_stats(0, 1)["pairs"]
     print("pairs per inner apply: with cells %.4g, without %.4g" % (p_cells, p_plain))
     assert p_cells < 0.97 * p_plain
@@ -302,7 +305,7 @@ def test_inner_operator_source_cells():
     ref.apply_lop(band, want, 1.0)
     err = rel(with_cells.apply_inner(band)[rows], want[rows])
     print("inner with cells vs oracle on sampled spheres %.3e" % err)
-    assert err < 1e-5
+    assert err < 2e-5
 
 
 def test_inner_level_orders_from_environment(monkeypatch):
@@ -322,15 +325,15 @@ def test_inner_level_orders_from_environment(monkeypatch):
     x = np.random.default_rng(5).standard_normal(op.n_dof)
     ref, y64 = op.apply_inner(x), op.apply(x)
     del op
-    for v in ("3,6,11,18:-1,-1,-1,-1", "3,6,11,18:8,5,4,3", "3,6,11,18"):
+    for v in ("3,6,9,16:-1,-1,-1,-1", "3,6,9,16:8,5,4,3", "3,6,9,16"):
         monkeypatch.setenv("SPHSTK_LEVEL_ALPHA", v)
         assert np.array_equal(inner(x), ref), v
-    monkeypatch.setenv("SPHSTK_LEVEL_ALPHA", "3,6,11,18:-1,-1,3,2")
+    monkeypatch.setenv("SPHSTK_LEVEL_ALPHA", "3,6,9,16:-1,-1,3,2")
     low = inner(x)
     print("lower far-level orders: change %.3e, error vs FP64 %.3e (default %.3e)" %
           (rel(low, ref), rel(low, y64), rel(ref, y64)))
     assert 0 < rel(low, ref) < 1e-3
-    for bad in ("3,6,11", "3,6,11,18:1,2", "3,6,11,18:-1,-1,-1,99"):
+    for bad in ("3,6,9", "3,6,9,16:1,2", "3,6,9,16:-1,-1,-1,99"):
         monkeypatch.setenv("SPHSTK_LEVEL_ALPHA", bad)
         with pytest.raises(S.SphstkError):
             inner(x)
