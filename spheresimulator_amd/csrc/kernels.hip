@@ -177,12 +177,45 @@ __device__ __forceinline__ void far_pair(const R sx, const R sy, const R sz, con
 
 // Sources [b, e) of an 8-word record array, four at a time so that the
 // wave-uniform (scalar) loads of a batch are issued before its arithmetic.
+// FP32 records: the loads of the next batch are issued before the arithmetic
+// of the current one, so their latency (the sources stream from beyond the
+// L2) hides behind it (16384 spheres p=12: 12% faster).  FP64 records keep
+// one batch in flight: two would not fit the wave's scalar registers.
 template <int KIND, int TPL, bool MASKED, typename R>
 __device__ __forceinline__ void far_range(const R *__restrict__ src, int64_t b, int64_t e, const R *x, const R *y,
                                           const R *z, const R *nx, const R *ny, const R *nz, const bool *incl, R *ax,
                                           R *ay, R *az) {
     int64_t s = b;
     const int64_t e4 = b + ((e - b) & ~int64_t(3));
+    if constexpr (sizeof(R) == 4) {
+        if (s != e4) {
+            R v[4][6];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int c = 0; c < 6; c++) v[u][c] = src[8 * (s + u) + c];
+            for (; s + 4 != e4; s += 4) {
+                R w[4][6];
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int c = 0; c < 6; c++) w[u][c] = src[8 * (s + 4 + u) + c];
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    far_pair<KIND, TPL, MASKED>(v[u][0], v[u][1], v[u][2], v[u][3], v[u][4], v[u][5], x, y, z, nx,
+                                                ny, nz, incl, ax, ay, az);
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int c = 0; c < 6; c++) v[u][c] = w[u][c];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                far_pair<KIND, TPL, MASKED>(v[u][0], v[u][1], v[u][2], v[u][3], v[u][4], v[u][5], x, y, z, nx, ny,
+                                            nz, incl, ax, ay, az);
+            s = e4;
+        }
+    }
     for (; s != e4; s += 4) {
         R v[4][6];
 #pragma unroll
