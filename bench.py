@@ -62,12 +62,16 @@ def solve_loop(M, force, steps, warmup, budget_end=None, fixed_after=0.0):
     before it."""
     zero = np.zeros(M.n_dof)
     req_steps, req_warm = steps, warmup
+    count = [0]
 
     def one():
         M.set_unknown(zero)
         t0 = time.perf_counter()
         v, info = M.solveMob(force)
-        return time.perf_counter() - t0, v, info
+        t = time.perf_counter() - t0
+        count[0] += 1
+        log("solve %d: %.3f s" % (count[0], t))  # a progress line per solve
+        return t, v, info
 
     times = []
     if warmup == 0:
@@ -238,9 +242,8 @@ def main():
     out["mixed_vs_fp64"] = {"speedup": t_fp64 / (ms / 1000.0), "rel_err_V": eV, "rel_err_Omega": eW}
     out["roofline"] = far_roofline(st32, FP32_VALU_PEAK, "k_far_f32<traction>: the mid-range list, 4 level lists "
                                    "and 2 source-cell lists of one FP32 inner apply", 7)
-    out["mixed"]["fp64_outer_far"] = far_roofline(st64ml, FP64_VALU_PEAK, "k_far_f64 (mid-range list) and "
-                                                  "k_far_f64_lvl (4 level lists) <traction> of one FP64 residual "
-                                                  "apply", 5)
+    out["mixed"]["fp64_outer_far"] = far_roofline(st64ml, FP64_VALU_PEAK, "k_far_f64_lvl<traction>: the mid-range "
+                                                  "list and 4 level lists of one FP64 residual apply", 5)
     if not args.no_cpu:
         log("CPU baseline (oracle, bounded sample)")
         out["cpu_baseline"] = cpu_baseline(c, fp64_applies)
