@@ -458,6 +458,7 @@ void Operator::autotune_inner(double &budget_s) {
 }
 
 Operator::~Operator() {
+    if (ls64_build_.valid()) ls64_build_.wait();
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (aux_) (void)hipStreamSynchronize(aux_);
     if (ev_fork_) (void)hipEventDestroy(ev_fork_);
@@ -1273,7 +1274,12 @@ void Operator::far_copies(int kind, double coef, const double *in, double *out, 
 // The FP64 multi-level far field is used when its pair evaluations (fine mid
 // range plus all levels) are at most 3/4 of the direct sum's; below that the
 // resampling and the extra launches eat the saving (many small suspensions).
+void Operator::wait_outer_levels() {
+    if (ls64_build_.valid()) ls64_build_.get();
+}
+
 bool Operator::outer_levels_pay() {
+    wait_outer_levels();
     if (outer_pay_ < 0) {
         ensure_levels(ls64_);
         double ml = ls64_.mid.pairs;
@@ -1477,9 +1483,22 @@ void Operator::ensure_centre() {
 }
 
 // Builds everything the mixed-precision solve uses (otherwise built on first
-// use): the FP32 and FP64 level sets and the final-velocity lists.
+// use): the FP32 and FP64 level sets and the final-velocity lists.  The FP64
+// set is first needed by the first refinement residual, after a whole inner
+// solve, so it is built on a host thread meanwhile: its construction is host
+// work, allocations and synchronous copies (no launches on the operator's
+// non-blocking stream), and nothing else reads ls64_ until
+// outer_levels_pay() has joined the thread (16384 spheres p=12: 3.1 s of the
+// 4.5 s set-up).
 void Operator::prepare_mixed(bool outer_levels) {
     if (nsph_ == 0) return;
+    if (outer_levels && !ls64_.ready && !ls64_build_.valid()) {
+        const int dev = device_;
+        ls64_build_ = std::async(std::launch::async, [this, dev] {
+            SPHSTK_HIP(hipSetDevice(dev));
+            ensure_levels(ls64_);
+        });
+    }
     if (multi_level_) {
         if (flags_ & 4) {
             double budget = kTuneBudget;
@@ -1487,7 +1506,6 @@ void Operator::prepare_mixed(bool outer_levels) {
         }
         ensure_levels(ls32_);
     }
-    if (outer_levels) outer_levels_pay();
     ensure_centre();
     SPHSTK_HIP(hipStreamSynchronize(stream_));
 }

@@ -6,6 +6,7 @@
 
 #include "kernels.hpp"
 
+#include <future>
 #include <memory>
 #include <string>
 
@@ -295,6 +296,8 @@ class Operator {
     LevelSet ls32_, ls64_;
     bool multi_level_ = true, timing_done_ = false;
     int outer_pay_ = -1; // FP64 multi-level outer far field in use: -1 undecided, 0 no, 1 yes
+    std::future<void> ls64_build_; // ls64_ being built on a host thread (prepare_mixed)
+    void wait_outer_levels();      // joins that build, rethrowing its error
 
     // captured inner-operator apply (SPHSTK_FLAG_GRAPH): g_in_ -> g_out_
     hipGraph_t g_inner_ = nullptr;

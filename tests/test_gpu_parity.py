@@ -360,6 +360,26 @@ def test_outer_fp64_operator_accuracy(p):
     assert 0 < ml < 0.75 * direct
 
 
+def test_mixed_outer_levels_built_in_background():
+    """The mixed form builds its FP64 outer level set on a host thread during
+    the set-up and the first inner solve: a mobility destroyed right after
+    creation joins that thread, and solves on a fresh mobility equal those on
+    one whose set-up finished before the first solve (the standalone operator
+    builds the set on first use)."""
+    say("mixed form: FP64 outer level set built in the background")
+    c = S.configs.config2()
+    M = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], precision=S.PREC_MIXED)
+    del M
+    M = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], precision=S.PREC_MIXED)
+    v1, info1 = M.solveMob(c["force"])
+    M.set_unknown(np.zeros(M.n_dof))
+    v2, info2 = M.solveMob(c["force"])
+    assert info1["converged"] == 1 and np.array_equal(v1, v2)
+    op = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+    x = np.random.default_rng(17).standard_normal(op.n_dof)
+    assert np.array_equal(op.apply_outer(x), M.op.apply_outer(x))
+
+
 def test_mixed_matches_fp64_low_order_polydisperse():
     """Mixed form vs FP64 form on polydisperse suspensions with low orders (the
     first suspensions of the batched workload: p in {6..16}, R in [0.5, 1.5],
