@@ -60,6 +60,8 @@ extern "C" {
                                           once twice the inner operator's measured relative error times the
                                           correction's size is below tol / 2, a step takes the inner solve's
                                           residual plus that bound) */
+#define SPHSTK_FLAG_NO_COARSE 128 /* mixed form: plain GMRES on the FP32 operator for the inner solves (default:
+                                     refinement with a coarse FP32 operator, FP32 residuals) */
 
 typedef struct sphstk_operator sphstk_operator;
 typedef struct sphstk_mobility sphstk_mobility;

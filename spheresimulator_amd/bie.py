@@ -24,6 +24,7 @@ FLAG_GRAPH = 8
 FLAG_INNER_PRECOND = 16
 FLAG_NO_CELLS = 32
 FLAG_EXPLICIT_REFINE = 64
+FLAG_NO_COARSE = 128
 
 
 def _d(a):
@@ -211,7 +212,7 @@ class SphereSTKMobMat:
 
     def __init__(self, pos, radius, quat, order, viscosity=1.0, counts=None, precision=PREC_FP64, params=None,
                  timing=False, fp32_direct=False, autotune=False, graph=False, inner_precond=False, cells=True,
-                 explicit_refine=False):
+                 explicit_refine=False, coarse=True):
         L = load()
         s = _Spheres(pos, radius, quat, order, counts)
         self.n_spheres = s.n
@@ -227,6 +228,7 @@ class SphereSTKMobMat:
                                                (FLAG_GRAPH if graph else 0) |
                                                (FLAG_INNER_PRECOND if inner_precond else 0) |
                                                (FLAG_EXPLICIT_REFINE if explicit_refine else 0) |
+                                               (0 if coarse else FLAG_NO_COARSE) |
                                                (0 if cells else FLAG_NO_CELLS), ctypes.byref(h)))
         self._h = h
         self.n_dof = self.op.n_dof

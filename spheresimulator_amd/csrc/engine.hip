@@ -75,8 +75,8 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     c_[1] = cSL;
     c_[2] = cTrac;
     c_[3] = cLOP;
-    SPHSTK_REQUIRE((flags & ~127) == 0, "unknown flags (known: TIMING 1, FP32_DIRECT 2, AUTOTUNE 4, GRAPH 8, "
-                                        "INNER_PRECOND 16, NO_CELLS 32, EXPLICIT_REFINE 64)");
+    SPHSTK_REQUIRE((flags & ~255) == 0, "unknown flags (known: TIMING 1, FP32_DIRECT 2, AUTOTUNE 4, GRAPH 8, "
+                                        "INNER_PRECOND 16, NO_CELLS 32, EXPLICIT_REFINE 64, NO_COARSE 128)");
     SPHSTK_REQUIRE(nsusp >= 1, "number of suspensions must be >= 1");
     SPHSTK_REQUIRE(ncopy >= 1 && ncopy <= 64 && nsusp % ncopy == 0, "copies must divide the suspensions");
     SPHSTK_REQUIRE(counts != nullptr, "sphere counts must not be NULL");
@@ -261,6 +261,17 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     ls64_.alo[0] = 7.0, ls64_.alo[1] = 14.0, ls64_.alo[2] = 28.0, ls64_.alo[3] = 56.0;
     ls64_.order[0] = 8, ls64_.order[1] = 6, ls64_.order[2] = 5, ls64_.order[3] = 4;
     ls64_.fp64 = true;
+    // coarse FP32 operator of the mixed form's preconditioning solves: orders
+    // 5, 4, 3, 2 from 2, 3, 5, 8 (R_s + R_t) (capped at p)
+    lsc_.alo[0] = 2.0, lsc_.alo[1] = 3.0, lsc_.alo[2] = 5.0, lsc_.alo[3] = 8.0;
+    lsc_.order[0] = 5, lsc_.order[1] = 4, lsc_.order[2] = 3, lsc_.order[3] = 2;
+    lsc_.fp64 = false;
+    lsc_.cells = ls32_.cells;
+    if (const char *v = std::getenv("SPHSTK_COARSE_ALPHA")) {
+        const int got = std::sscanf(v, "%lf,%lf,%lf,%lf:%d,%d,%d,%d", &lsc_.alo[0], &lsc_.alo[1], &lsc_.alo[2],
+                                    &lsc_.alo[3], &lsc_.order[0], &lsc_.order[1], &lsc_.order[2], &lsc_.order[3]);
+        if (got != 4 && got != 8) throw Error(-1, "SPHSTK_COARSE_ALPHA: a0,a1,a2,a3[:p0,p1,p2,p3]");
+    }
     // separations, optionally followed by orders (-1: the order-dependent default)
     if (const char *v = std::getenv("SPHSTK_LEVEL_ALPHA")) {
         const int got = std::sscanf(v, "%lf,%lf,%lf,%lf:%d,%d,%d,%d", &ls32_.alo[0], &ls32_.alo[1], &ls32_.alo[2],
@@ -268,7 +279,7 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
                                     &ls32_.order[3]);
         if (got != 4 && got != 8) throw Error(-1, "SPHSTK_LEVEL_ALPHA: a0,a1,a2,a3[:p0,p1,p2,p3]");
     }
-    for (const LevelSet *S : {&ls32_, &ls64_}) {
+    for (const LevelSet *S : {&ls32_, &ls64_, &lsc_}) {
         SPHSTK_REQUIRE(S->alo[0] >= 1.5, "level separations must start at >= 1.5");
         for (int l = 1; l < kNumLevels; l++)
             SPHSTK_REQUIRE(S->alo[l] > S->alo[l - 1], "level separations must increase");
@@ -311,7 +322,7 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
             build_far_lists(lists_fine_, nsusp_, susp_sph_off_, pt_off_, pt_off_, runs_fine_, tpl, split, kWantItems);
         }
     }
-    ls32_.tpl = ls64_.tpl = tpl;
+    ls32_.tpl = ls64_.tpl = lsc_.tpl = tpl;
 
     src64_.alloc(std::max<int64_t>(8 * npts_, 8));
     coef_.alloc(std::max<int64_t>(coef_off_[nsph_], 1));
@@ -459,6 +470,7 @@ void Operator::autotune_inner(double &budget_s) {
 
 Operator::~Operator() {
     if (ls64_build_.valid()) ls64_build_.wait();
+    if (ls32_build_.valid()) ls32_build_.wait();
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (aux_) (void)hipStreamSynchronize(aux_);
     if (ev_fork_) (void)hipEventDestroy(ev_fork_);
@@ -1277,6 +1289,9 @@ void Operator::far_copies(int kind, double coef, const double *in, double *out, 
 void Operator::wait_outer_levels() {
     if (ls64_build_.valid()) ls64_build_.get();
 }
+void Operator::wait_inner_levels() {
+    if (ls32_build_.valid()) ls32_build_.get();
+}
 
 bool Operator::outer_levels_pay() {
     wait_outer_levels();
@@ -1328,6 +1343,7 @@ void Operator::far(int kind, int prec, double coef, const double *in, double *ou
             timing_done_ = true;
         }
     } else {
+        if (prec == 1) wait_inner_levels();
         if (!trg32_.p) {
             trg32_.alloc(8 * std::max<int64_t>(npts_, 1));
             src32_.alloc(8 * std::max<int64_t>(npts_, 1));
@@ -1335,7 +1351,8 @@ void Operator::far(int kind, int prec, double coef, const double *in, double *ou
         }
         launch_pack_src_f32(npts_, d_coords_.p, d_w_.p, in, src32_.p, stream_);
         if (multi_level_) {
-            far_levels(ls32_, kind, coef, in, out, active, timing ? ev.a : nullptr, timing ? ev.b : nullptr);
+            far_levels(prec == 3 ? lsc_ : ls32_, kind, coef, in, out, active, timing ? ev.a : nullptr,
+                       timing ? ev.b : nullptr);
             if (timing) {
                 pending_.push_back(ev);
                 timing_done_ = true;
@@ -1490,7 +1507,7 @@ void Operator::ensure_centre() {
 // non-blocking stream), and nothing else reads ls64_ until
 // outer_levels_pay() has joined the thread (16384 spheres p=12: 3.1 s of the
 // 4.5 s set-up).
-void Operator::prepare_mixed(bool outer_levels) {
+void Operator::prepare_mixed(bool outer_levels, bool coarse) {
     if (nsph_ == 0) return;
     if (outer_levels && !ls64_.ready && !ls64_build_.valid()) {
         const int dev = device_;
@@ -1504,7 +1521,19 @@ void Operator::prepare_mixed(bool outer_levels) {
             double budget = kTuneBudget;
             autotune_inner(budget);
         }
-        ensure_levels(ls32_);
+        if (coarse && !ls32_.ready && !ls32_build_.valid()) {
+            // with the coarse operator first, A32 is first needed after the
+            // first coarse GMRES: its set is built beside it the same way (its
+            // one launch, the target packing, writes only ls32_'s buffers)
+            const int dev = device_;
+            ls32_build_ = std::async(std::launch::async, [this, dev] {
+                SPHSTK_HIP(hipSetDevice(dev));
+                ensure_levels(ls32_);
+            });
+        } else {
+            ensure_levels(ls32_);
+        }
+        if (coarse) ensure_levels(lsc_);
     }
     ensure_centre();
     SPHSTK_HIP(hipStreamSynchronize(stream_));
@@ -1715,12 +1744,17 @@ void Operator::near_pairs(int32_t *trg, int32_t *src) const {
 
 void Operator::kernel_stats(int kind, int prec, double *ms, int64_t *launches, double *pairs) {
     SPHSTK_REQUIRE(kind == 0 || kind == 1, "kind must be 0 (traction) or 1 (single layer)");
-    SPHSTK_REQUIRE(prec >= 0 && prec < kNumPrec, "prec must be 0 (FP64), 1 (FP32) or 2 (FP64 multi-level)");
+    SPHSTK_REQUIRE(prec >= 0 && prec < kNumPrec,
+                   "prec must be 0 (FP64), 1 (FP32), 2 (FP64 multi-level) or 3 (coarse FP32)");
+    if (prec == 1) wait_inner_levels();
     harvest_events();
     if (ms) *ms = stats_[kind][prec].ms;
     if (launches) *launches = stats_[kind][prec].launches;
     if (pairs) {
-        const LevelSet *S = (prec == 2 && outer_pay_ == 1) ? &ls64_ : (prec == 1 && multi_level_) ? &ls32_ : nullptr;
+        const LevelSet *S = (prec == 2 && outer_pay_ == 1) ? &ls64_
+                            : (prec == 1 && multi_level_) ? &ls32_
+                            : (prec == 3 && multi_level_) ? &lsc_
+                                                          : nullptr;
         if (S && S->ready) {
             *pairs = S->mid.pairs + (S->agg ? S->agg->pairs : 0.0);
             for (auto &L : S->levels) *pairs += L->lists.pairs;
@@ -1763,7 +1797,8 @@ Mobility::Mobility(int nsusp, const int *counts, const double *pos, const double
     act_.alloc(A_->nsusp());
     scal_.alloc(4 * (size_t)A_->nsusp());
     d_nact_.alloc(1);
-    if (prec_ == 1) A_->prepare_mixed(outer_prec_ == 2);
+    coarse_inner_ = prec_ == 1 && (flags & 128) == 0 && A_->has_coarse();
+    if (prec_ == 1) A_->prepare_mixed(outer_prec_ == 2, coarse_inner_);
     SPHSTK_HIP(hipStreamSynchronize(A_->stream()));
     // last, so that nothing after them can throw
     for (auto &e : nact_ev_) SPHSTK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1799,11 +1834,11 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
     Operator &A = *A_;
     hipStream_t st = A.stream();
     const int B = A.nsusp();
-    const int m = std::max(1, prec == 1 ? inner_restart_ : prm_.restart);
+    const int m = std::max(1, prec != 0 ? inner_restart_ : prm_.restart);
     const int64_t n = A.ndof();
     if (V_.n < size_t(m + 1) * n) V_.alloc(size_t(m + 1) * n);
     h_.alloc(size_t(2 * (m + 1) + 1) * B);
-    if (prec == 1 && m + 1 <= 64) mdot_.alloc(size_t(m + 1) * std::max(A.nchunks(), 1));
+    if (prec != 0 && m + 1 <= 64) mdot_.alloc(size_t(m + 1) * std::max(A.nchunks(), 1));
     coefv_.alloc(size_t(m + 1) * B);
     const Chunk *ch = A.chunks();
     const int nch = A.nchunks();
@@ -1811,7 +1846,7 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
     double *part = A.scratch_partial();
     // inner solve: right preconditioning by the isolated-sphere inverse,
     // A M^-1 u = r, d = M^-1 u (the residual stays the unpreconditioned one)
-    const bool rprec = prec == 1 && inner_precond_;
+    const bool rprec = (prec == 1 || prec == 3) && inner_precond_;
     if (rprec) {
         z_.alloc(std::max<int64_t>(n, 1));
         zt_.alloc(std::max<int64_t>(n, 1));
@@ -1882,9 +1917,14 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
             const double *Vj = V_.p + size_t(j) * n;
             if (rprec) {
                 A.precondition(Vj, z_.p, act_.p);
-                A.apply_inner_into(z_.p, W, act_.p);
+                if (prec == 3)
+                    A.apply_coarse_into(z_.p, W, act_.p);
+                else
+                    A.apply_inner_into(z_.p, W, act_.p);
             } else if (prec == 1)
                 A.apply_inner_into(Vj, W, act_.p);
+            else if (prec == 3)
+                A.apply_coarse_into(Vj, W, act_.p);
             else
                 A.apply_into(Vj, W, prec, act_.p);
             applies++;
@@ -1893,7 +1933,7 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
             // vector at a time.  Inner FP32-operator solve: CGS2 (classical
             // Gram-Schmidt twice: all j+1 products of a pass in one kernel),
             // which stores -h.
-            const bool cgs2 = prec == 1 && m + 1 <= 64;
+            const bool cgs2 = prec != 0 && m + 1 <= 64;
             for (int pass = 0; pass < 2; pass++) {
                 if (cgs2) {
                     double *hv = h_.p + size_t(pass * (m + 1)) * B;
@@ -1927,7 +1967,8 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
                 for (int s = 0; s < B; s++)
                     if (cyc[s]) worst = std::max(worst, rel[s]);
                 std::fprintf(stderr, "[sphstk] gmres(%s) iter %d: max implicit relres %.3e, %d/%d suspensions active\n",
-                             prec ? "fp32 inner" : "fp64", total, worst, nact_host_[j & 1], B);
+                             prec == 3 ? "fp32 coarse" : prec ? "fp32 inner" : "fp64", total, worst, nact_host_[j & 1],
+                             B);
                 std::fflush(stderr);
                 if (nact_host_[j & 1] == 0) break;
             }
@@ -1974,10 +2015,10 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
                 if (kdim[s] > 0) info[s].relres = rel[s];
             }
         // Belos' explicit residual test (yaml :12-13) for the FP64 solve.  The
-        // inner FP32-operator solve of the mixed form skips it once the
-        // implicit residual has converged: the refinement computes the FP64
-        // residual of the updated iterate right afterwards.
-        const bool explicit_final = prm_.explicit_check != 0 && prec != 1;
+        // FP32-operator solves of the mixed form skip it once the implicit
+        // residual has converged: the refinement computes the residual of the
+        // updated iterate right afterwards (or takes the solve's own).
+        const bool explicit_final = prm_.explicit_check != 0 && prec == 0;
         bool need_res = explicit_final;
         for (int s = 0; s < B; s++)
             if (cyc[s] && !impok[s]) need_res = true;
@@ -2000,6 +2041,101 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
             }
         }
     }
+}
+
+// The mixed form's inner solve A32 d = rhs (to itol, relative to |rhs|) by
+// refinement with the coarse FP32 operator Ac (the FP32 far field on the coarse
+// level set: about a third of A32's pair sums, relative error ~1e-2): d = 0,
+// r = rhs; repeat { GMRES on Ac e = r to max(kCoarseTol, tol needed); d += e;
+// r = rhs - A32 d (one A32 apply) } until |r| <= itol |rhs|.  Each step gains
+// about Ac's error (1e-2), the steps' Krylov spaces are those of the cheap
+// operator, and A32 enters only through the residuals.  If a step gains less
+// than 2x on some suspension, the rest is solved with A32 itself.  On return
+// res = rhs - A32 d, exact up to A32's rounding.  Without the coarse operator
+// (its flag, or an FP32 direct far field) this is GMRES on A32.
+constexpr double kCoarseTol = 5e-3;
+constexpr int kMaxCoarseSteps = 8;
+void Mobility::inner_solve(const double *rhs, double *d, double itol, std::vector<int> &act,
+                           std::vector<SegInfo> &info, int &app32, double *res) {
+    if (!coarse_inner_) {
+        gmres(rhs, d, true, itol, 1, act, info, app32, res);
+        return;
+    }
+    Operator &A = *A_;
+    hipStream_t st = A.stream();
+    const int B = A.nsusp();
+    const int64_t n = A.ndof();
+    ce_.alloc(std::max<int64_t>(n, 1));
+    ct_.alloc(std::max<int64_t>(n, 1));
+    std::vector<double> bn, rn;
+    seg_norms(rhs, bn);
+    launch_fill(n, 0.0, d, st);
+    launch_axpby(n, 1.0, rhs, 0.0, res, st);
+    rn = bn;
+    std::vector<int> a(B, 0), stalled(B, 0);
+    std::vector<double> ones(B, 1.0);
+    SPHSTK_HIP(hipMemcpyAsync(scal_.p + 3 * B, ones.data(), sizeof(double) * B, hipMemcpyHostToDevice, st));
+    auto live = [&]() {
+        bool any = false;
+        for (int s = 0; s < B; s++) {
+            a[s] = act[s] && bn[s] > 0.0 && rn[s] > itol * bn[s] && !stalled[s];
+            any = any || a[s];
+        }
+        return any;
+    };
+    for (int step = 0; step < kMaxCoarseSteps && live(); step++) {
+        double worst = 0.0;
+        for (int s = 0; s < B; s++)
+            if (a[s]) worst = std::max(worst, rn[s] / bn[s]);
+        const double ctol = std::max(kCoarseTol, 0.5 * itol / worst);
+        std::vector<SegInfo> sub(B);
+        std::vector<int> sa = a;
+        launch_fill(n, 0.0, ce_.p, st);
+        gmres(res, ce_.p, true, ctol, 3, sa, sub, app32, nullptr);
+        // d += e (e is zero outside this step's suspensions); ct = A32 d there;
+        // res = rhs - ct on those suspensions only
+        launch_axpby(n, 1.0, ce_.p, 1.0, d, st);
+        SPHSTK_HIP(hipMemcpyAsync(act_.p, a.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
+        A.apply_inner_into(d, ct_.p, act_.p);
+        app32++;
+        launch_seg_scale(res, rhs, scal_.p + 3 * B, A.chunks(), A.nchunks(), act_.p, st);
+        launch_seg_axpy(res, ct_.p, scal_.p + 3 * B, -1.0, A.chunks(), A.nchunks(), act_.p, st);
+        std::vector<double> prev = rn;
+        seg_norms(res, rn);
+        for (int s = 0; s < B; s++)
+            if (a[s]) {
+                info[s].iters += sub[s].iters;
+                if (rn[s] > 0.5 * prev[s]) stalled[s] = 1;
+            }
+        if (verbose_) {
+            double w = 0.0;
+            for (int s = 0; s < B; s++)
+                if (a[s]) w = std::max(w, rn[s] / bn[s]);
+            std::fprintf(stderr, "[sphstk] coarse step %d: tolerance %.2e, FP32 relres %.3e\n", step + 1, ctol, w);
+        }
+    }
+    // what is left (a stalled or unfinished suspension): GMRES on A32 for the
+    // rest of the correction, from a copy of the residual
+    bool rest = false;
+    for (int s = 0; s < B; s++) {
+        a[s] = act[s] && bn[s] > 0.0 && rn[s] > itol * bn[s];
+        rest = rest || a[s];
+    }
+    if (rest) {
+        double tmin = 1.0;
+        for (int s = 0; s < B; s++)
+            if (a[s]) tmin = std::min(tmin, itol * bn[s] / rn[s]);
+        std::vector<SegInfo> sub(B);
+        launch_axpby(n, 1.0, res, 0.0, ct_.p, st);
+        launch_fill(n, 0.0, ce_.p, st);
+        gmres(ct_.p, ce_.p, true, tmin, 1, a, sub, app32, res);
+        launch_axpby(n, 1.0, ce_.p, 1.0, d, st);
+        for (int s = 0; s < B; s++) info[s].iters += sub[s].iters;
+        seg_norms(res, rn);
+        if (verbose_) std::fprintf(stderr, "[sphstk] inner solve finished with the FP32 operator\n");
+    }
+    for (int s = 0; s < B; s++)
+        if (act[s] && bn[s] > 0.0) info[s].relres = rn[s] / bn[s];
 }
 
 // solveMob (BIE/SphereSTKMobMat.cpp:146-286)
@@ -2099,7 +2235,7 @@ void Mobility::solve(const double *force, double *vel, std::vector<SegInfo> &inf
             // the inner solve overwrites r_, so its right-hand side is a copy of the outer residual
             gmres_inner_rhs_.alloc(n);
             launch_axpby(n, 1.0, r_.p, 0.0, gmres_inner_rhs_.p, st);
-            gmres(gmres_inner_rhs_.p, d_.p, true, itol, 1, ia, inner, app32, rin_.p);
+            inner_solve(gmres_inner_rhs_.p, d_.p, itol, ia, inner, app32, rin_.p);
             SPHSTK_HIP(hipMemcpyAsync(act_.p, a2.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
             std::vector<double> ones(B, 1.0);
             SPHSTK_HIP(hipMemcpyAsync(scal_.p + 2 * B, ones.data(), sizeof(double) * B, hipMemcpyHostToDevice, st));

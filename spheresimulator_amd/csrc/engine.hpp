@@ -196,7 +196,7 @@ class Operator {
     // one int per suspension or nullptr for all.
     // out += cId in + cSL S[in] + cTrac K'[in].  Far field: prec 0 FP64 direct
     // sum, 1 FP32 (multi-level unless disabled), 2 FP64 multi-level.
-    static constexpr int kNumPrec = 3;
+    static constexpr int kNumPrec = 4;
     void p2p(const double *in, double *out, double cId, double cSL, double cTrac, int prec, const int *active);
     // out += cLOP L[in]
     void lop(const double *in, double *out, double cLOP, const int *active);
@@ -207,6 +207,9 @@ class Operator {
     // out = A32 in (the FP32 inner operator); with SPHSTK_FLAG_GRAPH replayed
     // from a HIP graph captured on the first call for this `active` array
     void apply_inner_into(const double *in, double *out, const int *active);
+    // the coarse FP32 operator Ac (prec 3): the FP32 operator on the coarse level set lsc_
+    void apply_coarse_into(const double *in, double *out, const int *active) { apply_into(in, out, 3, active); }
+    bool has_coarse() const { return multi_level_; }
     void project(double *inout);
     // out = M^-1 in with M the per-sphere self operator (exact inverse per sphere)
     void precondition(const double *in, double *out, const int *active);
@@ -215,7 +218,7 @@ class Operator {
     // rigid-body velocities (6 per sphere) of the single layer of rho, distant pairs by Faxen's laws
     void rigid_velocity(const double *rho, double inv_mu, double *vel);
     // builds the level sets and final-velocity lists of the mixed-precision solve now
-    void prepare_mixed(bool outer_levels);
+    void prepare_mixed(bool outer_levels, bool coarse);
     // out += cId rho0 + cTrac K'[rho0] for the rigid-body density rho0 of `force`
     // (6 per sphere): self/near and pairs closer than 4 (R_s + R_t) as p2p, distant
     // pairs in closed form at the sphere centres
@@ -294,10 +297,13 @@ class Operator {
     std::vector<Runs> runs_fine_; // source runs of the direct far field per target sphere
     std::string tune_log_;         // autotuner decisions
     LevelSet ls32_, ls64_;
+    LevelSet lsc_; // coarse FP32 operator (prec 3): the preconditioning solves of the mixed form
     bool multi_level_ = true, timing_done_ = false;
     int outer_pay_ = -1; // FP64 multi-level outer far field in use: -1 undecided, 0 no, 1 yes
     std::future<void> ls64_build_; // ls64_ being built on a host thread (prepare_mixed)
     void wait_outer_levels();      // joins that build, rethrowing its error
+    std::future<void> ls32_build_; // ls32_ likewise, when the coarse operator runs first
+    void wait_inner_levels();
 
     // captured inner-operator apply (SPHSTK_FLAG_GRAPH): g_in_ -> g_out_
     hipGraph_t g_inner_ = nullptr;
@@ -375,6 +381,10 @@ class Mobility {
     int prec_;
     int outer_prec_ = 2; // far field of the mixed form's FP64 applies: 2 multi-level, 0 direct
     int inner_restart_ = 30; // GMRES restart of the inner FP32-operator solve
+    bool coarse_inner_ = false; // inner solves by refinement with the coarse operator Ac
+    DevBuf<double> ce_, ct_;    // coarse correction and A32 d of those solves
+    void inner_solve(const double *rhs, double *d, double itol, std::vector<int> &act, std::vector<SegInfo> &info,
+                     int &app32, double *res);
     bool inner_precond_ = false; // right-precondition the inner solve by the isolated-sphere inverse
     DevBuf<double> z_, zt_;      // preconditioned basis vector / update
     bool x_zero_ = true; // the stored unknown is known to be zero (cold start)

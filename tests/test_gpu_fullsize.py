@@ -170,3 +170,30 @@ def test_fullsize_mixed_residual_update_against_explicit_steps(cfg2):
     res = float(np.linalg.norm(rhs - op.apply(out[False][2])) / np.linalg.norm(rhs))
     print("FP64 direct residual of the density: %.3e (reported bound %.3e)" % (res, out[False][1]["relres"]))
     assert res <= 1e-8
+
+
+def test_fullsize_mixed_coarse_inner_against_plain_inner(cfg2):
+    """Inner solves by refinement with the coarse FP32 operator (the default)
+    against plain GMRES on the FP32 operator (SPHSTK_FLAG_NO_COARSE): both
+    converge, the velocities agree to the tolerance's scale, and the FP64
+    direct-sum residual of the default's density is below the tolerance."""
+    c = cfg2["c"]
+    say("1024 spheres p=12: mixed solve, coarse-operator inner solves vs plain")
+    out = {}
+    for coarse in (True, False):
+        M = S.SphereSTKMobMat(c["pos"], c["radius"], c["quat"], c["order"], precision=S.PREC_MIXED, coarse=coarse)
+        v, info = M.solveMob(c["force"])
+        print("coarse=%d: outer %d Krylov iterations %d FP32 applies %d relres %.3e" % (
+            coarse, info["outer"], info["iters"], info["applies_fp32"], info["relres"]))
+        assert info["converged"] == 1 and info["relres"] <= 1e-8
+        out[coarse] = (v, M.get_unknown())
+        del M
+    a, b = out[True][0].reshape(-1, 6), out[False][0].reshape(-1, 6)
+    eV, eW = rel(a[:, :3], b[:, :3]), rel(a[:, 3:], b[:, 3:])
+    print("coarse vs plain inner solves: rel diff V %.3e Omega %.3e" % (eV, eW))
+    assert eV < 1e-8 and eW < 1e-8
+    op = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+    rhs = op.applyP2POP(rho0_of(cfg2), np.zeros(op.n_dof), -0.5, 0.0, -1.0)
+    res = float(np.linalg.norm(rhs - op.apply(out[True][1])) / np.linalg.norm(rhs))
+    print("FP64 direct residual of the density: %.3e" % res)
+    assert res <= 1e-8
