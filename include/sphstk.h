@@ -187,7 +187,8 @@ int sphstk_operator_far_shape(const sphstk_operator *op, int *tpl, int *split, i
 /* Far-field kernel timing collected with SPHSTK_FLAG_TIMING: total milliseconds,
  * launches, and the algorithmic pair count per launch, for the traction (kind 0)
  * or single-layer (kind 1) pair sums of the FP64 operator (prec 0), the FP32
- * inner operator (prec 1) or the FP64 multi-level outer operator (prec 2). */
+ * inner operator (prec 1), the FP64 multi-level outer operator (prec 2) or the
+ * coarse FP32 operator of the mixed form's inner solves (prec 3). */
 int sphstk_operator_kernel_stats(const sphstk_operator *op, int kind, int prec, double *total_ms, int64_t *launches,
                                  double *pairs_per_launch);
 int sphstk_operator_reset_stats(sphstk_operator *op);

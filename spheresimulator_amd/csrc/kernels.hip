@@ -913,12 +913,19 @@ __device__ __forceinline__ void traction_from(const double *A, const double x[3]
 // 1: traction, 2: synthesis of the projected density (self items only).
 // MODE 0/1 write the item's contribution (lab frame, coefficients applied) to
 // scratch[slot + 3*lane]; MODE 2 writes out directly.
+// Workgroups go to the 8 XCDs round-robin, so workgroup b takes item
+// (b % 8) * ceil(nitems / 8) + b / 8: each XCD runs a contiguous range of the
+// items, whose chunks and neighbouring targets read the same source records
+// (63 KB at p = 12) from its own L2.  The grid is 8 ceil(nitems / 8).
 template <int MODE>
 __global__ __launch_bounds__(64) void k_sheval(DevGeom g, const double *__restrict__ rec,
-                                               const EvalItem *__restrict__ items, double *__restrict__ scratch,
-                                               double *__restrict__ out, double cSL, double cTrac,
-                                               const int *__restrict__ active) {
-    const EvalItem it = items[blockIdx.x];
+                                               const EvalItem *__restrict__ items, int nitems,
+                                               double *__restrict__ scratch, double *__restrict__ out, double cSL,
+                                               double cTrac, const int *__restrict__ active) {
+    const int chunk = (int)(gridDim.x >> 3);
+    const int pos = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
+    if (pos >= nitems) return;
+    const EvalItem it = items[pos];
     const int T = it.trg, S = it.src;
     if (active && !active[g.sph_susp[T]]) return;
     const int lane = threadIdx.x;
@@ -993,10 +1000,11 @@ __global__ __launch_bounds__(256) void k_sheval_gather(DevGeom g, const int64_t 
 void launch_sheval_items(const DevGeom &g, int mode, const double *rec, const EvalItem *items, int nitems,
                          double *scratch, double cSL, double cTrac, const int *active, hipStream_t s) {
     if (!g.nsph || nitems <= 0) return;
+    const dim3 grid(8 * ((nitems + 7) / 8));
     if (mode == 0)
-        k_sheval<0><<<dim3(nitems), dim3(64), 0, s>>>(g, rec, items, scratch, nullptr, cSL, cTrac, active);
+        k_sheval<0><<<grid, dim3(64), 0, s>>>(g, rec, items, nitems, scratch, nullptr, cSL, cTrac, active);
     else
-        k_sheval<1><<<dim3(nitems), dim3(64), 0, s>>>(g, rec, items, scratch, nullptr, cSL, cTrac, active);
+        k_sheval<1><<<grid, dim3(64), 0, s>>>(g, rec, items, nitems, scratch, nullptr, cSL, cTrac, active);
 }
 
 void launch_sheval_gather(const DevGeom &g, int nitems, const int64_t *trg_slot_off, const int *trg_nsrc,
@@ -1016,7 +1024,8 @@ void launch_sheval(const DevGeom &g, int mode, const double *rec, const EvalItem
 void launch_synthesis(const DevGeom &g, const double *rec, const EvalItem *self_items, int nitems, double *out,
                       hipStream_t s, const int *active) {
     if (!g.nsph || nitems <= 0) return;
-    k_sheval<2><<<dim3(nitems), dim3(64), 0, s>>>(g, rec, self_items, nullptr, out, 0.0, 0.0, active);
+    k_sheval<2><<<dim3(8 * ((nitems + 7) / 8)), dim3(64), 0, s>>>(g, rec, self_items, nitems, nullptr, out, 0.0, 0.0,
+                                                                  active);
 }
 
 __global__ void k_identity(DevGeom g, const double *__restrict__ in, double *__restrict__ out, double cId,
