@@ -2053,7 +2053,7 @@ void Mobility::gmres(const double *b, double *x, bool x_zero, double tol, int pr
 // than 2x on some suspension, the rest is solved with A32 itself.  On return
 // res = rhs - A32 d, exact up to A32's rounding.  Without the coarse operator
 // (its flag, or an FP32 direct far field) this is GMRES on A32.
-constexpr double kCoarseTol = 5e-3;
+constexpr double kCoarseTol = 1e-2;
 constexpr int kMaxCoarseSteps = 8;
 void Mobility::inner_solve(const double *rhs, double *d, double itol, std::vector<int> &act,
                            std::vector<SegInfo> &info, int &app32, double *res) {
