@@ -143,6 +143,14 @@ int sphstk_operator_apply_inner(sphstk_operator *op, const double *X, double *Y,
  * A64; relative error about 1e-9 against A64. */
 int sphstk_operator_apply_outer(sphstk_operator *op, const double *X, double *Y, double alpha, double beta);
 
+/* The coarse FP32 operator of the mixed-precision solve's inner solves (their
+ * refinement steps run GMRES on it), Y := beta Y + alpha Ac X (one column).  No
+ * counterpart in the reference: as the FP32 inner operator, with coarse grids of
+ * order 5, 4, 3 and 2 (capped at p) from 2, 3, 5 and 8 (R_s + R_t) on
+ * (SPHSTK_COARSE_ALPHA); relative error about 1e-2 against A64; unless
+ * SPHSTK_FLAG_FP32_DIRECT was given at creation, in which case it is A32. */
+int sphstk_operator_apply_coarse(sphstk_operator *op, const double *X, double *Y, double alpha, double beta);
+
 /* applyP2POP: out += cId in + cSL S[in] + cTrac K'[in]  (cpp:262-485; hpp:204-206). */
 int sphstk_operator_apply_p2p(sphstk_operator *op, const double *in, double *out, double cId, double cSL,
                               double cTrac);

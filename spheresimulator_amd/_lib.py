@@ -19,7 +19,7 @@ EXPORTED = [
     "sphstk_last_error", "sphstk_version", "sphstk_device_count",
     "sphstk_operator_create", "sphstk_operator_create_batched", "sphstk_operator_destroy",
     "sphstk_operator_sizes", "sphstk_operator_apply", "sphstk_operator_apply_device", "sphstk_operator_apply_inner",
-    "sphstk_operator_apply_outer",
+    "sphstk_operator_apply_outer", "sphstk_operator_apply_coarse",
     "sphstk_operator_apply_p2p", "sphstk_operator_apply_p2p_device", "sphstk_operator_apply_lop",
     "sphstk_operator_project_nullspace", "sphstk_operator_apply_self_inverse", "sphstk_operator_pole_values", "sphstk_operator_get_grid", "sphstk_operator_get_near_pairs",
     "sphstk_operator_far_shape", "sphstk_operator_kernel_stats", "sphstk_operator_reset_stats",
@@ -82,6 +82,7 @@ def load():
     L.sphstk_operator_apply_device.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.sphstk_operator_apply_inner.argtypes = [_vp, _dp, _dp, ctypes.c_double, ctypes.c_double]
     L.sphstk_operator_apply_outer.argtypes = [_vp, _dp, _dp, ctypes.c_double, ctypes.c_double]
+    L.sphstk_operator_apply_coarse.argtypes = [_vp, _dp, _dp, ctypes.c_double, ctypes.c_double]
     L.sphstk_operator_pole_values.argtypes = [_vp, _dp, _dp]
     L.sphstk_operator_far_shape.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                            ctypes.POINTER(ctypes.c_int)]

@@ -117,6 +117,13 @@ class SphereSTKSHOperator:
         check(load().sphstk_operator_apply_outer(self._h, _d(X), _d(Y), alpha, beta))
         return Y
 
+    def apply_coarse(self, X, Y=None, alpha=1.0, beta=0.0):
+        """The coarse FP32 operator of the mixed-precision solve's inner solves (one column)."""
+        X = _f64(X, self.n_dof, "X")
+        Y = np.zeros(self.n_dof) if Y is None else _out(Y, self.n_dof, "Y")
+        check(load().sphstk_operator_apply_coarse(self._h, _d(X), _d(Y), alpha, beta))
+        return Y
+
     def applyP2POP(self, inp, out, cId, cSL, cTrac):
         """out += cId in + cSL S[in] + cTrac K'[in]  (accumulates, as the reference)."""
         inp = _f64(inp, self.n_dof, "in")

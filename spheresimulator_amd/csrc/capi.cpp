@@ -231,6 +231,10 @@ int sphstk_operator_apply_outer(sphstk_operator *op, const double *X, double *Y,
     return apply_prec(op, X, Y, alpha, beta, 2);
 }
 
+int sphstk_operator_apply_coarse(sphstk_operator *op, const double *X, double *Y, double alpha, double beta) {
+    return apply_prec(op, X, Y, alpha, beta, 3);
+}
+
 int sphstk_operator_apply_p2p(sphstk_operator *op, const double *in, double *out, double cId, double cSL,
                               double cTrac) {
     return guard([&] {

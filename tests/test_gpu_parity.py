@@ -270,6 +270,28 @@ def test_inner_fp32_operator_accuracy(p, direct, tol):
     assert err < tol
 
 
+def test_coarse_fp32_operator_accuracy():
+    """The coarse FP32 operator of the mixed form's inner solves (coarse grids
+    of order 5, 4, 3, 2 from 2, 3, 5, 8 (R_s + R_t)) approximates the FP64
+    operator to about 1e-2 on a band-limited density: coarser than the FP32
+    inner operator, fine enough that each refinement step gains about 1e-2.
+    With the FP32 direct far field it is the FP32 inner operator itself."""
+    say("coarse FP32 operator (256 spheres p=12)")
+    c = S.configs.make_suspension(256, 12, 31, vf=0.1, min_gap=0.2)
+    gpu = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0)
+    x = gpu.projectNullSpace(np.random.default_rng(8).standard_normal(gpu.n_dof))
+    y64 = gpu.apply(x)
+    e_c, e_32 = rel(gpu.apply_coarse(x), y64), rel(gpu.apply_inner(x), y64)
+    print("coarse rel err %.3e, inner rel err %.3e" % (e_c, e_32))
+    assert e_32 < e_c < 3e-2
+    pairs_c, pairs_32 = gpu.kernel_stats(0, 3)["pairs"], gpu.kernel_stats(0, 1)["pairs"]
+    print("pair evaluations per apply: coarse %.4g, inner %.4g" % (pairs_c, pairs_32))
+    assert 0 < pairs_c < 0.6 * pairs_32
+    direct = S.SphereSTKSHOperator(c["pos"], c["radius"], c["quat"], c["order"], 0.5, 0.0, 1.0, 1.0,
+                                   fp32_direct=True)
+    assert np.array_equal(direct.apply_coarse(x), direct.apply_inner(x))
+
+
 def test_inner_operator_source_cells():
     """Source-cell aggregation of the FP32 inner operator's two coarsest levels
     (cells of 16 spheres, the leaves of a k-d split, as 200 equivalent point
