@@ -100,6 +100,7 @@ struct EquivTables {
     double re = 0.0, rc = 0.0;
     std::vector<double> eq, chk, P;
 };
-const EquivTables &equiv_tables();
+constexpr int kEquivK = 200; // equivalent point forces per cell of the FP32 operator
+const EquivTables &equiv_tables(int K = kEquivK);
 
 } // namespace sphstk

@@ -262,11 +262,14 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     ls64_.order[0] = 8, ls64_.order[1] = 6, ls64_.order[2] = 5, ls64_.order[3] = 4;
     ls64_.fp64 = true;
     // coarse FP32 operator of the mixed form's preconditioning solves: orders
-    // 5, 4, 3, 2 from 2, 3, 5, 8 (R_s + R_t) (capped at p)
+    // 5, 4, 3, 2 from 2, 3, 5, 8 (R_s + R_t) (capped at p), source cells of 32
+    // equivalent point forces from 2.5 cell radii (on the check sphere)
     lsc_.alo[0] = 2.0, lsc_.alo[1] = 3.0, lsc_.alo[2] = 5.0, lsc_.alo[3] = 8.0;
     lsc_.order[0] = 5, lsc_.order[1] = 4, lsc_.order[2] = 3, lsc_.order[3] = 2;
     lsc_.fp64 = false;
     lsc_.cells = ls32_.cells;
+    lsc_.cell_theta = 2.5;
+    lsc_.equiv_k = 32;
     if (const char *v = std::getenv("SPHSTK_COARSE_ALPHA")) {
         const int got = std::sscanf(v, "%lf,%lf,%lf,%lf:%d,%d,%d,%d", &lsc_.alo[0], &lsc_.alo[1], &lsc_.alo[2],
                                     &lsc_.alo[3], &lsc_.order[0], &lsc_.order[1], &lsc_.order[2], &lsc_.order[3]);
@@ -907,7 +910,7 @@ void Operator::ensure_levels(LevelSet &S) {
     // and per target the pair evaluations its cell interactions save
     std::vector<std::array<Runs, 2>> crun(S.cells ? nsph_ : 0);
     std::vector<double> saved(S.cells ? nsph_ : 0, 0.0);
-    const int Keq = equiv_tables().K;
+    const int Keq = equiv_tables(S.equiv_k).K;
     auto scan = [&](int ta, int tb) {
         std::vector<int> ml;
         // pair class of (t, s): -3 self, -2 near, -1 mid, else the level.  The
@@ -954,7 +957,7 @@ void Operator::ensure_levels(LevelSet &S) {
                 for (int64_t c = susp_cell_off[b]; c < susp_cell_off[b + 1]; c++) {
                     const double ex = ccen[3 * c] - pos_[3 * t], ey = ccen[3 * c + 1] - pos_[3 * t + 1],
                                  ez = ccen[3 * c + 2] - pos_[3 * t + 2];
-                    if (std::sqrt(ex * ex + ey * ey + ez * ez) >= kCellTheta * crho[c] + rad_[t]) {
+                    if (std::sqrt(ex * ex + ey * ey + ez * ez) >= S.cell_theta * crho[c] + rad_[t]) {
                         int lmin = kNumLevels;
                         double member_pairs = 0.0;
                         for (int r = cell_r0[c]; r < cell_r1[c] && lmin >= 2; r++) {
@@ -989,7 +992,7 @@ void Operator::ensure_levels(LevelSet &S) {
             for (const Runs &rr : cr)
                 for (const auto &r : rr)
                     for (int c = r.first; c < r.second; c++) used[c] = 1;
-        const EquivTables &E = equiv_tables();
+        const EquivTables &E = equiv_tables(S.equiv_k);
         for (int c = 0; c < ncell; c++)
             if (used[c]) {
                 double pts = 0.0;
@@ -1123,9 +1126,10 @@ void Operator::ensure_levels(LevelSet &S) {
         const int ncell = nused;
         PhaseTimer pc("  cells");
         auto A = std::make_unique<CellAgg>();
-        const EquivTables &E = equiv_tables();
+        const EquivTables &E = equiv_tables(S.equiv_k);
         const int K = E.K, Kc = E.Kc, m = 3 * Kc, n = 3 * K;
         A->ncell = ncell;
+        A->E = &E;
         A->susp_cell_off = susp_cell_off;
         std::vector<float> chk(8 * (size_t)ncell * Kc, 0.f), eq(8 * (size_t)ncell * K, 0.f);
         std::vector<int> csusp(ncell);
@@ -1219,10 +1223,10 @@ void Operator::far_levels(LevelSet &S, int kind, double coef, const double *in, 
     if (A) {
         // check velocities of each cell's raw fine-grid sources (single layer,
         // FP32 pairs, FP64 sums), then the equivalent point forces
-        launch_fill((int64_t)A->ncell * 3 * equiv_tables().Kc, 0.0, A->chk_u.p, stream_);
+        launch_fill((int64_t)A->ncell * 3 * A->E->Kc, 0.0, A->chk_u.p, stream_);
         far_run(A->chk_lists, 1, 1, src32_.p, nullptr, nullptr, A->chk32.p, A->chk_chunks.p, (int)A->chk_chunks.n,
                 1.0 / (8.0 * kPi), A->chk_u.p, active);
-        launch_equiv_forces(A->ncell, equiv_tables().K, equiv_tables().Kc, A->P.p, A->chk_u.p, A->rho.p, A->susp.p,
+        launch_equiv_forces(A->ncell, A->E->K, A->E->Kc, A->P.p, A->chk_u.p, A->rho.p, A->susp.p,
                             active, A->eq32.p, stream_);
     }
     if (ev_a) SPHSTK_HIP(hipEventRecord(ev_a, stream_));

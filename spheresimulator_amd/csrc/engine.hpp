@@ -117,14 +117,15 @@ struct FarLevel {
 // Source-cell aggregation of the two coarsest FP32 levels (the treecode part
 // of the far field): cells of kCellSize spheres (the leaves of a balanced k-d
 // split of each suspension's centres), each
-// represented beyond kCellTheta cell radii by the K equivalent point forces of
-// equiv_tables(); a (target sphere, cell) pair whose members all lie in
+// represented beyond cell_theta cell radii (kCellTheta on the FP32 operator)
+// by the equiv_k equivalent point forces of equiv_tables(equiv_k); a (target sphere, cell) pair whose members all lie in
 // levels 2-3 of the target interacts through them on the target's grid of the
 // finer of its members' levels.
 constexpr int kCellSize = 16;
 constexpr double kCellTheta = 4.0;
 struct CellAgg {
     int ncell = 0;
+    const EquivTables *E = nullptr;   // the set's equivalent-source template
     double pairs = 0.0;               // pair evaluations of the two cell lists (per apply)
     double check_pairs = 0.0;         // pair evaluations of the check velocities (per apply)
     std::vector<int64_t> susp_cell_off; // first cell of each suspension
@@ -150,6 +151,8 @@ struct LevelSet {
     std::vector<std::unique_ptr<FarLevel>> levels;
     FarLists mid;
     bool cells = false;               // aggregate sources over cells on levels 2-3
+    double cell_theta = kCellTheta;   // cells serve targets beyond cell_theta cell radii
+    int equiv_k = kEquivK;            // equivalent point forces per cell
     std::unique_ptr<CellAgg> agg;
     bool ready = false;
 };

@@ -15,11 +15,13 @@
 // (nearly uniform; the Gauss-Legendre grids cluster at the poles and fit an
 // order of magnitude worse).  With K = 200, Kc = 400, re = 1.1, rc = 2.5,
 // 16 spheres of radius rho/8 inside the cell and targets at 4 rho, velocity
-// and traction are reproduced to about 1e-7 and 1.2e-6 (relative).
+// and traction are reproduced to about 1e-7 and 1.2e-6 (relative).  Smaller
+// templates (Kc = 2K, same radii) trade accuracy for pair evaluations.
 #include "common.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <mutex>
 #include <thread>
 
@@ -54,10 +56,10 @@ void for_columns(int n, F body) {
     for (auto &x : th) x.join();
 }
 
-EquivTables build() {
+EquivTables build(int K) {
     EquivTables T;
-    T.K = 200;
-    T.Kc = 400;
+    T.K = K;
+    T.Kc = 2 * K;
     T.re = 1.1;
     T.rc = 2.5;
     fibonacci_sphere(T.K, T.eq);
@@ -124,11 +126,15 @@ EquivTables build() {
 
 } // namespace
 
-const EquivTables &equiv_tables() {
-    static std::once_flag once;
-    static EquivTables T;
-    std::call_once(once, [] { T = build(); });
-    return T;
+const EquivTables &equiv_tables(int K) {
+    SPHSTK_REQUIRE(K >= 16 && K <= 400, "equivalent-source template size out of range");
+    // built once per size; entries of the map never move
+    static std::mutex mu;
+    static std::map<int, EquivTables> tabs;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = tabs.find(K);
+    if (it == tabs.end()) it = tabs.emplace(K, build(K)).first;
+    return it->second;
 }
 
 } // namespace sphstk

@@ -1,15 +1,19 @@
-// Host-side check of the equivalent-source template of the far-field cell
+// Host-side check of the equivalent-source templates of the far-field cell
 // aggregation (equiv.cpp): P is a left inverse of the unit cell's Stokeslet
 // matrix, and the fitted point forces reproduce the velocity and the traction
 // of 16 clustered sphere grids (radius 1/8 inside the unit cell, random
-// forces) at 4 cell radii to the accuracy the inner operator needs, with the
-// total force (the flow's leading Stokeslet term) preserved.
+// forces), with the total force (the flow's leading Stokeslet term) preserved.
+// Argument K: 200 (the FP32 operator's template, targets at 4 cell radii, to
+// the accuracy that operator needs) or 32 (the coarse operator's, targets on
+// its check sphere at 2.5 cell radii, to a few 1e-3 in velocity: white-noise
+// forces are its worst case).
 #include "common.hpp"
 
 #include <cmath>
 #include <cstdio>
 #include <array>
 #include <random>
+#include <cstdlib>
 
 using namespace sphstk;
 
@@ -24,8 +28,12 @@ static void traction(const double *r, const double *n, const double *f, double *
     for (int a = 0; a < 3; a++) t[a] += -3.0 / (4 * kPi) * r[a] * rf * rn * std::pow(ri, 5);
 }
 
-int main() {
-    const EquivTables &T = equiv_tables();
+int main(int argc, char **argv) {
+    const int K = argc > 1 ? std::atoi(argv[1]) : kEquivK;
+    const bool coarse = K == 32;
+    if (K != kEquivK && !coarse) return 2;
+    const double dist = coarse ? 2.5 : 4.0;
+    const EquivTables &T = equiv_tables(K);
     const int n = 3 * T.K, m = 3 * T.Kc;
     int fails = 0;
     // P G = I
@@ -90,7 +98,7 @@ int main() {
         const double lx = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
         const double ln = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
         for (int c = 0; c < 3; c++) {
-            x[c] *= 4.0 / lx;
+            x[c] *= dist / lx;
             nn[c] /= ln;
         }
         double ue[3] = {0, 0, 0}, ud[3] = {0, 0, 0}, te[3] = {0, 0, 0}, td[3] = {0, 0, 0};
@@ -114,8 +122,11 @@ int main() {
         }
     }
     const double eu = std::sqrt(du / nu), et = std::sqrt(dt / nt);
-    std::printf("16 clustered sphere grids, targets at 4 rho: velocity %.2e traction %.2e total force %.2e\n", eu,
-                et, eF);
-    fails += !(eu < 1e-6 && et < 5e-6 && eF < 1e-8);
+    std::printf("K = %d, 16 clustered sphere grids, targets at %.1f rho: velocity %.2e traction %.2e total force "
+                "%.2e\n", K, dist, eu, et, eF);
+    if (coarse)
+        fails += !(eu < 5e-3 && et < 4e-2 && eF < 2e-4);
+    else
+        fails += !(eu < 1e-6 && et < 5e-6 && eF < 1e-8);
     return fails ? 1 : 0;
 }
