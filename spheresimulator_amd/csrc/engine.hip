@@ -252,14 +252,16 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     partial_.alloc(std::max<size_t>(ch.size(), 1));
 
     int tpl = 1, split = 0;
-    // FP32 inner operator: orders from level_order(); FP64 outer applies: fine
-    // below 7 (R_s + R_t), then orders 8, 6, 5, 4 from 7, 14, 28, 56 (R_s + R_t).
+    // FP32 inner operator: orders from level_order().  FP64 outer applies: fine
+    // below s' = 5.8, then orders 9, 7, 6, 5 from s' = 5.8, 10, 15, 26 (the
+    // radius-aware separation of ensure_levels()), where a level's worst
+    // truncation ratio (2 s' - 1)^-(q + 1) is 5.6e-11 to 5.9e-11.
     ls32_.alo[0] = 3.0, ls32_.alo[1] = 6.0, ls32_.alo[2] = 9.0, ls32_.alo[3] = 16.0;
     ls32_.order[0] = ls32_.order[1] = ls32_.order[2] = ls32_.order[3] = -1;
     ls32_.fp64 = false;
     ls32_.cells = (flags & 32) == 0;
-    ls64_.alo[0] = 7.0, ls64_.alo[1] = 14.0, ls64_.alo[2] = 28.0, ls64_.alo[3] = 56.0;
-    ls64_.order[0] = 8, ls64_.order[1] = 6, ls64_.order[2] = 5, ls64_.order[3] = 4;
+    ls64_.alo[0] = 5.8, ls64_.alo[1] = 10.0, ls64_.alo[2] = 15.0, ls64_.alo[3] = 26.0;
+    ls64_.order[0] = 9, ls64_.order[1] = 7, ls64_.order[2] = 6, ls64_.order[3] = 5;
     ls64_.fp64 = true;
     // coarse FP32 operator of the mixed form's preconditioning solves: orders
     // 5, 4, 3, 2 from 2, 3, 5, 8 (R_s + R_t) (capped at p), source cells of 32

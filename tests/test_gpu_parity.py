@@ -364,10 +364,10 @@ def test_inner_level_orders_from_environment(monkeypatch):
 @pytest.mark.parametrize("p", [10, 12])
 def test_outer_fp64_operator_accuracy(p):
     """The FP64 multi-level outer operator of the mixed form (pairs beyond
-    7 (R_s + R_t) through grids of order min(p, 8/6/5/4)) matches the FP64 direct
+    s' = 5.8 through grids of order min(p, 9/7/6/5)) matches the FP64 direct
     operator on a band-limited density and on a white one (no band limit), to
     far below the 1e-8 GMRES tolerance the refinement residual serves (about
-    8e-10 at 1024 spheres, p=12).  The
+    3e-10 here, p=12).  The
     dilute suspension (vf 0.02) makes the levels pay, so they are in use."""
     say("outer FP64 multi-level operator (p=%d)" % p)
     c = S.configs.make_suspension(256, p, 33, vf=0.02, min_gap=0.2)
