@@ -162,9 +162,11 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     }
 
     // self/near evaluation work items: per target sphere its own expansion first,
-    // then its near sources in list order, each cut into chunks of 64 targets
+    // then its near sources in list order, each cut into chunks of 64 targets;
+    // the same items grouped by source for the traction evaluation
     {
         std::vector<EvalItem> items, selfi;
+        std::vector<std::vector<EvalItem>> by_src(nsph_);
         std::vector<int64_t> tso(nsph_ + 1, 0);
         std::vector<int> tns(nsph_, 0);
         int64_t slot = 0;
@@ -177,6 +179,7 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
                 for (int j = 0; j < nT; j += 64) {
                     EvalItem it{t, s, j, std::min(64, nT - j), slot + 3 * (int64_t)j};
                     items.push_back(it);
+                    by_src[s].push_back(it);
                     if (k == 0) selfi.push_back(it);
                 }
                 slot += 3 * (int64_t)nT;
@@ -185,6 +188,18 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
         tso[nsph_] = slot;
         items_.upload(items);
         self_items_.upload(selfi);
+        std::vector<EvalItem> si;
+        std::vector<int64_t> soff(nsph_ + 1, 0);
+        si.reserve(items.size());
+        for (int q = 0; q < nsph_; q++) {
+            si.insert(si.end(), by_src[q].begin(), by_src[q].end());
+            soff[q + 1] = (int64_t)si.size();
+        }
+        d_src_items_.upload(si);
+        d_src_off_.upload(soff);
+        src_items_.items = d_src_items_.p;
+        src_items_.src_off = d_src_off_.p;
+        src_items_.max_order = max_order_;
         d_trg_slot_off_.upload(tso);
         d_trg_nsrc_.upload(tns);
         evscratch_.alloc(std::max<int64_t>(slot, 1));
@@ -1395,7 +1410,8 @@ void Operator::p2p(const double *in, double *out, double cId, double cSL, double
         SPHSTK_HIP(hipStreamWaitEvent(aux_, ev_fork_, 0));
         launch_analysis(geom_, in, coef_.p, active, aux_, max_order_);
         launch_expand(geom_, 1, coef_.p, rec_.p, active, aux_, max_order_);
-        launch_sheval_items(geom_, 1, rec_.p, items_.p, (int)items_.n, evscratch_.p, 0.0, cTrac, active, aux_);
+        launch_sheval_items(geom_, 1, rec_.p, items_.p, (int)items_.n, evscratch_.p, 0.0, cTrac, active, aux_,
+                            &src_items_);
         SPHSTK_HIP(hipEventRecord(ev_join_, aux_));
         far(0, prec, -3.0 / (4.0 * kPi) * cTrac, in, out, active);
         SPHSTK_HIP(hipStreamWaitEvent(stream_, ev_join_, 0));
@@ -1409,7 +1425,7 @@ void Operator::p2p(const double *in, double *out, double cId, double cSL, double
         far(0, prec, -3.0 / (4.0 * kPi) * cTrac, in, out, active);
         launch_expand(geom_, 1, coef_.p, rec_.p, active, stream_, max_order_);
         launch_sheval(geom_, 1, rec_.p, items_.p, (int)items_.n, d_trg_slot_off_.p, d_trg_nsrc_.p, evscratch_.p, in,
-                      out, 0.0, cTrac, cid, active, stream_);
+                      out, 0.0, cTrac, cid, active, stream_, &src_items_);
         cid = 0.0;
     }
     if (doSL) {
@@ -1556,7 +1572,7 @@ void Operator::rigid_p2p(const double *force, const double *rho0, double *out, d
                      lists_rigid_.order.p, lists_rigid_.groups.p, lists_rigid_.iv.p, out, cTrac, 1, stream_);
     launch_expand(geom_, 1, coef_.p, rec_.p, nullptr, stream_, max_order_);
     launch_sheval(geom_, 1, rec_.p, items_.p, (int)items_.n, d_trg_slot_off_.p, d_trg_nsrc_.p, evscratch_.p, rho0, out,
-                  0.0, cTrac, cId, nullptr, stream_);
+                  0.0, cTrac, cId, nullptr, stream_, &src_items_);
     SPHSTK_HIP(hipGetLastError());
 }
 

@@ -323,6 +323,9 @@ class Operator {
 
     // self/near evaluation work items and their scratch
     DevBuf<EvalItem> items_, self_items_;
+    DevBuf<EvalItem> d_src_items_;    // items_ grouped by source sphere
+    DevBuf<int64_t> d_src_off_;
+    SrcItems src_items_;
     DevBuf<int64_t> d_trg_slot_off_;
     DevBuf<int> d_trg_nsrc_;
     DevBuf<double> evscratch_;

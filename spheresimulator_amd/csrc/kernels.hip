@@ -14,8 +14,8 @@
 //                            and resampling between grids, 16 spheres per tile.
 // Expansion   k_expand       coefficients -> exterior solid-harmonic expansion
 //                            records (the flow of SCTL StokesEvalSL/KL/KSelf).
-// Near/self   k_sheval       self (SphereSTKSHOperator.cpp:305-351, :375-456)
-//                            and near-pair (:575-695) SH evaluations.
+// Near/self   k_sheval_src,  self (SphereSTKSHOperator.cpp:305-351, :375-456)
+//             k_sheval       and near-pair (:575-695) SH evaluations.
 // LOP         k_lop          rigid-body projector (:487-521).
 // BLAS-1      k_seg_*        segmented (one segment per suspension) GMRES
 //                            vector kernels.
@@ -910,9 +910,9 @@ __device__ __forceinline__ void traction_from(const double *A, const double x[3]
 
 // One work item = one 64-lane wave: targets [tstart, tstart+tcount) of sphere T
 // against the expansion of sphere S (S == T: self).  MODE 0: single layer,
-// 1: traction, 2: synthesis of the projected density (self items only).
-// MODE 0/1 write the item's contribution (lab frame, coefficients applied) to
-// scratch[slot + 3*lane]; MODE 2 writes out directly.
+// 2: synthesis of the projected density (self items only); the traction runs
+// in k_sheval_src.  MODE 0 writes the item's contribution (lab frame,
+// coefficients applied) to scratch[slot + 3*lane]; MODE 2 writes out directly.
 // Workgroups go to the 8 XCDs round-robin, so workgroup b takes item
 // (b % 8) * ceil(nitems / 8) + b / 8: each XCD runs a contiguous range of the
 // items, whose chunks and neighbouring targets read the same source records
@@ -949,24 +949,6 @@ __global__ __launch_bounds__(64) void k_sheval(DevGeom g, const double *__restri
         for (int c = 0; c < 3; c++) res[c] = A[c] + r2 * A[3 + c] + xi[c] * A[6] + xg4[c];
         qrot(q, res, lab, false);
         for (int c = 0; c < 3; c++) scratch[it.slot + 3 * lane + c] = cSL * R * lab[c];
-    } else if (MODE == 1) {
-        const double Nl[3] = {g.norms[3 * gp], g.norms[3 * gp + 1], g.norms[3 * gp + 2]};
-        double vb[3];
-        qrot(q, Nl, vb, true);
-        if (S == T) {
-            double A[kRecTrac];
-            eval_record<kRecTrac, 0, kRecTrac>(recS, P, xi[0], xi[1], xi[2], A);
-            traction_from(A, xi, vb, res);
-            double xq[3];
-            cross3(xi, A + 30, xq);
-            for (int c = 0; c < 3; c++) res[c] += 0.5 * (A[26] * xi[c] + A[27 + c] + xq[c]);
-        } else {
-            double A[kRecTracNear];
-            eval_record<kRecTrac, 0, kRecTracNear>(recS, P, xi[0], xi[1], xi[2], A);
-            traction_from(A, xi, vb, res);
-        }
-        qrot(q, res, lab, false);
-        for (int c = 0; c < 3; c++) scratch[it.slot + 3 * lane + c] = cTrac * lab[c];
     } else {
         double A[7];
         eval_record<kRecTrac, 26, 7>(recS, P, xi[0], xi[1], xi[2], A);
@@ -975,6 +957,104 @@ __global__ __launch_bounds__(64) void k_sheval(DevGeom g, const double *__restri
         for (int c = 0; c < 3; c++) res[c] = A[0] * xi[c] + A[1 + c] + xq[c];
         qrot(q, res, lab, false);
         for (int c = 0; c < 3; c++) out[3 * gp + c] = lab[c];
+    }
+}
+
+// Traction evaluation of the self and near items with the source's record in
+// LDS: one workgroup of kSrcWaves waves per source sphere S runs all items
+// whose source is S (its self item and its near targets' chunks), the waves
+// taking the items in turn.  The record (kRecTrac complex arrays per term,
+// (p+3)(p+4)/2 terms: 63 KB at p = 12) is read from global memory once per
+// source instead of once per item through the scalar cache, whose loads each
+// term of the item loop waited on; with up to 256 VGPRs per lane (2 waves per
+// SIMD, which the LDS allows anyway) each term's LDS reads issue together.
+// Same arithmetic in the same order as a per-item evaluation: bit-identical.
+// Workgroup b takes source (b % 8) ceil(N / 8) + b / 8 (XCD-contiguous, as
+// k_sheval).
+constexpr int kSrcWaves = 4;
+
+template <int KCNT>
+__device__ __forceinline__ void eval_lds(const double2 *__restrict__ srec, int P, double x, double y, double z,
+                                         double *acc) {
+#pragma unroll
+    for (int a = 0; a < KCNT; a++) acc[a] = 0.0;
+    const double r2 = x * x + y * y + z * z;
+    const double ir2 = 1.0 / r2;
+    double dmr = rsqrt_f64(r2), dmi = 0.0; // D_m^m
+    for (int m = 0; m <= P; m++) {
+        if (m > 0) {
+            const double f = -(2.0 * m - 1.0) * ir2;
+            const double nr = f * (dmr * x - dmi * y), ni = f * (dmr * y + dmi * x);
+            dmr = nr;
+            dmi = ni;
+        }
+        double pr = 0.0, pi = 0.0, cr = dmr, ci = dmi;
+        const double mm = (double)m * m;
+        for (int n = m; n <= P; n++) {
+            const double2 *c = srec + kRecTrac * dtri(n, m);
+#pragma unroll
+            for (int a = 0; a < KCNT; a++) {
+                const double2 v = c[a];
+                acc[a] = __builtin_fma(v.x, cr, __builtin_fma(v.y, ci, acc[a]));
+            }
+            const double f1 = -(2.0 * n + 1.0) * z * ir2, f2 = -((double)n * n - mm) * ir2;
+            const double nr = __builtin_fma(f1, cr, f2 * pr), ni = __builtin_fma(f1, ci, f2 * pi);
+            pr = cr;
+            pi = ci;
+            cr = nr;
+            ci = ni;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_sheval_src(DevGeom g, const double *__restrict__ rec,
+                                                              const EvalItem *__restrict__ items,
+                                                              const int64_t *__restrict__ src_off,
+                                                              double *__restrict__ scratch, double cTrac,
+                                                              const int *__restrict__ active) {
+    extern __shared__ double2 srec[];
+    const int chunk = (int)(gridDim.x >> 3);
+    const int S = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
+    if (S >= g.nsph) return;
+    // near pairs lie within one suspension: the test is uniform over the workgroup
+    if (active && !active[g.sph_susp[S]]) return;
+    const int64_t i0 = src_off[S], i1 = src_off[S + 1];
+    if (i0 == i1) return;
+    const int P = g.order[S] + 2;
+    const int nrec = kRecTrac * ((P + 1) * (P + 2) / 2);
+    const double2 *recS = reinterpret_cast<const double2 *>(rec + g.rec_off[S]);
+    for (int i = threadIdx.x; i < nrec; i += blockDim.x) srec[i] = recS[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const double *q = g.quat + 4 * S;
+    const double R = g.rad[S];
+    const double iR = 1.0 / R;
+    const double cx = g.pos[3 * S], cy = g.pos[3 * S + 1], cz = g.pos[3 * S + 2];
+    for (int64_t k = i0 + (threadIdx.x >> 6); k < i1; k += kSrcWaves) {
+        const EvalItem it = items[k];
+        if (lane >= it.tcount) continue;
+        const int T = it.trg;
+        const int64_t gp = g.pt_off[T] + it.tstart + lane;
+        const double d[3] = {(g.coords[3 * gp] - cx) * iR, (g.coords[3 * gp + 1] - cy) * iR,
+                             (g.coords[3 * gp + 2] - cz) * iR};
+        const double Nl[3] = {g.norms[3 * gp], g.norms[3 * gp + 1], g.norms[3 * gp + 2]};
+        double xi[3], vb[3], res[3], lab[3];
+        qrot(q, d, xi, true);
+        qrot(q, Nl, vb, true);
+        if (T == S) {
+            double A[kRecTrac];
+            eval_lds<kRecTrac>(srec, P, xi[0], xi[1], xi[2], A);
+            traction_from(A, xi, vb, res);
+            double xq[3];
+            cross3(xi, A + 30, xq);
+            for (int c = 0; c < 3; c++) res[c] += 0.5 * (A[26] * xi[c] + A[27 + c] + xq[c]);
+        } else {
+            double A[kRecTracNear];
+            eval_lds<kRecTracNear>(srec, P, xi[0], xi[1], xi[2], A);
+            traction_from(A, xi, vb, res);
+        }
+        qrot(q, res, lab, false);
+        for (int c = 0; c < 3; c++) scratch[it.slot + 3 * lane + c] = cTrac * lab[c];
     }
 }
 
@@ -998,13 +1078,26 @@ __global__ __launch_bounds__(256) void k_sheval_gather(DevGeom g, const int64_t 
 }
 
 void launch_sheval_items(const DevGeom &g, int mode, const double *rec, const EvalItem *items, int nitems,
-                         double *scratch, double cSL, double cTrac, const int *active, hipStream_t s) {
+                         double *scratch, double cSL, double cTrac, const int *active, hipStream_t s,
+                         const SrcItems *by_src) {
     if (!g.nsph || nitems <= 0) return;
-    const dim3 grid(8 * ((nitems + 7) / 8));
-    if (mode == 0)
-        k_sheval<0><<<grid, dim3(64), 0, s>>>(g, rec, items, nitems, scratch, nullptr, cSL, cTrac, active);
-    else
-        k_sheval<1><<<grid, dim3(64), 0, s>>>(g, rec, items, nitems, scratch, nullptr, cSL, cTrac, active);
+    if (mode == 1) {
+        if (!by_src) throw Error(-2, "internal: the traction evaluation needs the items grouped by source");
+        const int P = by_src->max_order + 2;
+        const size_t lds = sizeof(double2) * kRecTrac * ((P + 1) * (P + 2) / 2);
+        static size_t lds_set = 0;
+        if (lds > 65536 && lds > lds_set) {
+            if (hipFuncSetAttribute((const void *)k_sheval_src, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds) != hipSuccess)
+                throw Error(-2, "cannot raise the LDS limit of the self/near evaluation to " + std::to_string(lds));
+            lds_set = lds;
+        }
+        k_sheval_src<<<dim3(8 * ((g.nsph + 7) / 8)), dim3(64 * kSrcWaves), lds, s>>>(
+            g, rec, by_src->items, by_src->src_off, scratch, cTrac, active);
+        return;
+    }
+    k_sheval<0><<<dim3(8 * ((nitems + 7) / 8)), dim3(64), 0, s>>>(g, rec, items, nitems, scratch, nullptr, cSL,
+                                                                   cTrac, active);
 }
 
 void launch_sheval_gather(const DevGeom &g, int nitems, const int64_t *trg_slot_off, const int *trg_nsrc,
@@ -1016,8 +1109,8 @@ void launch_sheval_gather(const DevGeom &g, int nitems, const int64_t *trg_slot_
 
 void launch_sheval(const DevGeom &g, int mode, const double *rec, const EvalItem *items, int nitems,
                    const int64_t *trg_slot_off, const int *trg_nsrc, double *scratch, const double *in, double *out,
-                   double cSL, double cTrac, double cId, const int *active, hipStream_t s) {
-    launch_sheval_items(g, mode, rec, items, nitems, scratch, cSL, cTrac, active, s);
+                   double cSL, double cTrac, double cId, const int *active, hipStream_t s, const SrcItems *by_src) {
+    launch_sheval_items(g, mode, rec, items, nitems, scratch, cSL, cTrac, active, s, by_src);
     launch_sheval_gather(g, nitems, trg_slot_off, trg_nsrc, scratch, in, out, cId, active, s);
 }
 

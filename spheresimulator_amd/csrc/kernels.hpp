@@ -97,14 +97,25 @@ struct EvalItem {
     int64_t slot;
 };
 
+// The same items grouped by source sphere (its self item first): source S
+// owns items [src_off[S], src_off[S + 1]).  The traction evaluation (mode 1)
+// runs one workgroup per source with the source's record staged in LDS.
+struct SrcItems {
+    const EvalItem *items = nullptr;
+    const int64_t *src_off = nullptr;
+    int max_order = 0; // largest sphere order (sizes the LDS record)
+};
+
 // out += cSL*S_self/near + cTrac*K_self/near + cId*in   (mode 0: single layer, 1: traction)
 void launch_sheval(const DevGeom &g, int mode, const double *rec, const EvalItem *items, int nitems,
                    const int64_t *trg_slot_off, const int *trg_nsrc, double *scratch, const double *in, double *out,
-                   double cSL, double cTrac, double cId, const int *active, hipStream_t s);
+                   double cSL, double cTrac, double cId, const int *active, hipStream_t s,
+                   const SrcItems *by_src = nullptr);
 // the two halves of launch_sheval: the per-item evaluations into scratch, and
 // the per-target gather (out += sum of the target's items + cId in)
 void launch_sheval_items(const DevGeom &g, int mode, const double *rec, const EvalItem *items, int nitems,
-                         double *scratch, double cSL, double cTrac, const int *active, hipStream_t s);
+                         double *scratch, double cSL, double cTrac, const int *active, hipStream_t s,
+                         const SrcItems *by_src = nullptr);
 void launch_sheval_gather(const DevGeom &g, int nitems, const int64_t *trg_slot_off, const int *trg_nsrc,
                           const double *scratch, const double *in, double *out, double cId, const int *active,
                           hipStream_t s);
