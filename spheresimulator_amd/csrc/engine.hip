@@ -195,11 +195,20 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
             si.insert(si.end(), by_src[q].begin(), by_src[q].end());
             soff[q + 1] = (int64_t)si.size();
         }
+        std::vector<int> sl;
+        sl.reserve(nsph_);
+        for (int p = 0; p <= kMaxOrder; p++) {
+            src_items_.order_off[p] = (int)sl.size();
+            for (int q = 0; q < nsph_; q++)
+                if (order_[q] == p) sl.push_back(q);
+        }
+        src_items_.order_off[kMaxOrder + 1] = (int)sl.size();
         d_src_items_.upload(si);
         d_src_off_.upload(soff);
+        d_src_list_.upload(sl);
         src_items_.items = d_src_items_.p;
         src_items_.src_off = d_src_off_.p;
-        src_items_.max_order = max_order_;
+        src_items_.srcs = d_src_list_.p;
         d_trg_slot_off_.upload(tso);
         d_trg_nsrc_.upload(tns);
         evscratch_.alloc(std::max<int64_t>(slot, 1));

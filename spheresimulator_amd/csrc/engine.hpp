@@ -325,6 +325,7 @@ class Operator {
     DevBuf<EvalItem> items_, self_items_;
     DevBuf<EvalItem> d_src_items_;    // items_ grouped by source sphere
     DevBuf<int64_t> d_src_off_;
+    DevBuf<int> d_src_list_;          // spheres by order
     SrcItems src_items_;
     DevBuf<int64_t> d_trg_slot_off_;
     DevBuf<int> d_trg_nsrc_;

@@ -969,8 +969,9 @@ __global__ __launch_bounds__(64) void k_sheval(DevGeom g, const double *__restri
 // term of the item loop waited on; with up to 256 VGPRs per lane (2 waves per
 // SIMD, which the LDS allows anyway) each term's LDS reads issue together.
 // Same arithmetic in the same order as a per-item evaluation: bit-identical.
-// Workgroup b takes source (b % 8) ceil(N / 8) + b / 8 (XCD-contiguous, as
-// k_sheval).
+// One launch per sphere order over that order's spheres (in k-d order), LDS
+// sized to its record; workgroup b takes list entry (b % 8) ceil(n / 8) + b / 8
+// (XCD-contiguous, as k_sheval).
 constexpr int kSrcWaves = 4;
 
 template <int KCNT>
@@ -1010,12 +1011,14 @@ __device__ __forceinline__ void eval_lds(const double2 *__restrict__ srec, int P
 __global__ __launch_bounds__(64 * kSrcWaves) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_sheval_src(DevGeom g, const double *__restrict__ rec,
                                                               const EvalItem *__restrict__ items,
                                                               const int64_t *__restrict__ src_off,
+                                                              const int *__restrict__ srcs, int nsrc,
                                                               double *__restrict__ scratch, double cTrac,
                                                               const int *__restrict__ active) {
     extern __shared__ double2 srec[];
     const int chunk = (int)(gridDim.x >> 3);
-    const int S = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
-    if (S >= g.nsph) return;
+    const int pos = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
+    if (pos >= nsrc) return;
+    const int S = srcs[pos];
     // near pairs lie within one suspension: the test is uniform over the workgroup
     if (active && !active[g.sph_susp[S]]) return;
     const int64_t i0 = src_off[S], i1 = src_off[S + 1];
@@ -1083,17 +1086,22 @@ void launch_sheval_items(const DevGeom &g, int mode, const double *rec, const Ev
     if (!g.nsph || nitems <= 0) return;
     if (mode == 1) {
         if (!by_src) throw Error(-2, "internal: the traction evaluation needs the items grouped by source");
-        const int P = by_src->max_order + 2;
-        const size_t lds = sizeof(double2) * kRecTrac * ((P + 1) * (P + 2) / 2);
-        static size_t lds_set = 0;
-        if (lds > 65536 && lds > lds_set) {
-            if (hipFuncSetAttribute((const void *)k_sheval_src, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds) != hipSuccess)
-                throw Error(-2, "cannot raise the LDS limit of the self/near evaluation to " + std::to_string(lds));
-            lds_set = lds;
+        for (int p = 0; p <= kMaxOrder; p++) {
+            const int a = by_src->order_off[p], n = by_src->order_off[p + 1] - a;
+            if (n <= 0) continue;
+            const int P = p + 2;
+            const size_t lds = sizeof(double2) * kRecTrac * ((P + 1) * (P + 2) / 2);
+            static size_t lds_set = 0;
+            if (lds > 65536 && lds > lds_set) {
+                if (hipFuncSetAttribute((const void *)k_sheval_src, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lds) != hipSuccess)
+                    throw Error(-2, "cannot raise the LDS limit of the self/near evaluation to " +
+                                        std::to_string(lds));
+                lds_set = lds;
+            }
+            k_sheval_src<<<dim3(8 * ((n + 7) / 8)), dim3(64 * kSrcWaves), lds, s>>>(
+                g, rec, by_src->items, by_src->src_off, by_src->srcs + a, n, scratch, cTrac, active);
         }
-        k_sheval_src<<<dim3(8 * ((g.nsph + 7) / 8)), dim3(64 * kSrcWaves), lds, s>>>(
-            g, rec, by_src->items, by_src->src_off, scratch, cTrac, active);
         return;
     }
     k_sheval<0><<<dim3(8 * ((nitems + 7) / 8)), dim3(64), 0, s>>>(g, rec, items, nitems, scratch, nullptr, cSL,

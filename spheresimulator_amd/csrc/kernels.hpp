@@ -97,13 +97,16 @@ struct EvalItem {
     int64_t slot;
 };
 
-// The same items grouped by source sphere (its self item first): source S
-// owns items [src_off[S], src_off[S + 1]).  The traction evaluation (mode 1)
-// runs one workgroup per source with the source's record staged in LDS.
+// The same items grouped by source sphere: source S owns items
+// [src_off[S], src_off[S + 1]).  The traction evaluation (mode 1) runs one
+// workgroup per source with the source's record staged in LDS, one launch per
+// sphere order (LDS sized to that order's record): srcs lists the spheres by
+// order, those of order p at [order_off[p], order_off[p + 1]).
 struct SrcItems {
     const EvalItem *items = nullptr;
     const int64_t *src_off = nullptr;
-    int max_order = 0; // largest sphere order (sizes the LDS record)
+    const int *srcs = nullptr;
+    int order_off[kMaxOrder + 2] = {};
 };
 
 // out += cSL*S_self/near + cTrac*K_self/near + cId*in   (mode 0: single layer, 1: traction)
