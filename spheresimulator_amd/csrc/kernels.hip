@@ -968,7 +968,8 @@ __global__ __launch_bounds__(64) void k_sheval(DevGeom g, const double *__restri
 // source instead of once per item through the scalar cache, whose loads each
 // term of the item loop waited on; with up to 256 VGPRs per lane (2 waves per
 // SIMD, which the LDS allows anyway) each term's LDS reads issue together.
-// Same arithmetic in the same order as a per-item evaluation: bit-identical.
+// eval_lds is eval_record with the coefficients read from LDS: the same
+// operations in the same order.
 // One launch per sphere order over that order's spheres (in k-d order), LDS
 // sized to its record; workgroup b takes list entry (b % 8) ceil(n / 8) + b / 8
 // (XCD-contiguous, as k_sheval).
