@@ -253,7 +253,7 @@ def main():
     out["roofline"]["coarse"] = far_roofline(stc, FP32_VALU_PEAK, "k_far_f32<traction>, coarse operator", 7)
     out["roofline"]["inner"] = far_roofline(st32, FP32_VALU_PEAK, "k_far_f32<traction>, FP32 operator", 7)
     out["mixed"]["fp64_outer_far"] = far_roofline(st64ml, FP64_VALU_PEAK, "k_far_f64_lvl<traction>: the mid-range "
-                                                  "list and 4 level lists of one FP64 residual apply", 5)
+                                                  "list and 6 level lists of one FP64 residual apply", 7)
     if not args.no_cpu:
         log("CPU baseline (oracle, bounded sample)")
         out["cpu_baseline"] = cpu_baseline(c, fp64_applies)

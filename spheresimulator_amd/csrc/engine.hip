@@ -277,15 +277,20 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
 
     int tpl = 1, split = 0;
     // FP32 inner operator: orders from level_order().  FP64 outer applies: fine
-    // below s' = 5.8, then orders 9, 7, 6, 5 from s' = 5.8, 10, 15, 26 (the
-    // radius-aware separation of ensure_levels()), where a level's worst
-    // truncation ratio (2 s' - 1)^-(q + 1) is 5.6e-11 to 5.9e-11.
+    // below s' = 4.8, then orders 10, 9, 8, 7, 6, 5 from s' = 4.8, 5.8, 7.35,
+    // 10.05, 15.1, 26 (the radius-aware separation of ensure_levels()), where a
+    // level's worst truncation ratio (2 s' - 1)^-(q + 1) is 5.6e-11 to 5.9e-11.
     ls32_.alo[0] = 3.0, ls32_.alo[1] = 6.0, ls32_.alo[2] = 9.0, ls32_.alo[3] = 16.0;
     ls32_.order[0] = ls32_.order[1] = ls32_.order[2] = ls32_.order[3] = -1;
     ls32_.fp64 = false;
     ls32_.cells = (flags & 32) == 0;
-    ls64_.alo[0] = 5.8, ls64_.alo[1] = 10.0, ls64_.alo[2] = 15.0, ls64_.alo[3] = 26.0;
-    ls64_.order[0] = 9, ls64_.order[1] = 7, ls64_.order[2] = 6, ls64_.order[3] = 5;
+    ls64_.nlev = 6;
+    {
+        const double a[6] = {4.8, 5.8, 7.35, 10.05, 15.1, 26.0};
+        const int q[6] = {10, 9, 8, 7, 6, 5};
+        std::copy(a, a + 6, ls64_.alo);
+        std::copy(q, q + 6, ls64_.order);
+    }
     ls64_.fp64 = true;
     // coarse FP32 operator of the mixed form's preconditioning solves: orders
     // 5, 4, 3, 2 from 2, 3, 5, 8 (R_s + R_t) (capped at p), source cells of 32
@@ -310,9 +315,11 @@ Operator::Operator(int nsusp, const int *counts, const double *pos, const double
     }
     for (const LevelSet *S : {&ls32_, &ls64_, &lsc_}) {
         SPHSTK_REQUIRE(S->alo[0] >= 1.5, "level separations must start at >= 1.5");
-        for (int l = 1; l < kNumLevels; l++)
+        SPHSTK_REQUIRE(S->nlev >= 1 && S->nlev <= kMaxLevels && (!S->cells || S->nlev == kNumLevels),
+                       "internal: level count");
+        for (int l = 1; l < S->nlev; l++)
             SPHSTK_REQUIRE(S->alo[l] > S->alo[l - 1], "level separations must increase");
-        for (int l = 0; l < kNumLevels; l++)
+        for (int l = 0; l < S->nlev; l++)
             SPHSTK_REQUIRE(S->order[l] == -1 || (S->order[l] >= 1 && S->order[l] <= kMaxOrder),
                            "level orders must lie in [1, max order]");
     }
@@ -472,8 +479,9 @@ void Operator::autotune_inner(double &budget_s) {
         if (budget_s < 0) break;
         LevelSet keep = std::move(ls32_);
         ls32_.tpl = tpl;
-        std::copy(keep.alo, keep.alo + kNumLevels, ls32_.alo);
-        std::copy(keep.order, keep.order + kNumLevels, ls32_.order);
+        ls32_.nlev = keep.nlev;
+        std::copy(keep.alo, keep.alo + kMaxLevels, ls32_.alo);
+        std::copy(keep.order, keep.order + kMaxLevels, ls32_.order);
         ls32_.fp64 = false;
         ls32_.cells = keep.cells;
         ls32_.ready = false;
@@ -894,7 +902,8 @@ void Operator::ensure_levels(LevelSet &S) {
     // mid runs in sphere index space.
     auto tn = std::make_unique<PhaseTimer>("  level pair classes");
     std::vector<Runs> mid(nsph_);
-    std::vector<std::vector<Runs>> lvl(kNumLevels, std::vector<Runs>(nsph_));
+    const int nlev = S.nlev;
+    std::vector<std::vector<Runs>> lvl(nlev, std::vector<Runs>(nsph_));
     std::vector<int> sph_s0(nsph_), sph_s1(nsph_);
     for (int b = 0; b < nsusp_; b++)
         for (int64_t t = susp_sph_off_[b]; t < susp_sph_off_[b + 1]; t++) {
@@ -964,7 +973,7 @@ void Operator::ensure_levels(LevelSet &S) {
             }
             if (sep < a[0]) return -1;
             int l = 0;
-            while (l + 1 < kNumLevels && sep >= a[l + 1]) l++;
+            while (l + 1 < nlev && sep >= a[l + 1]) l++;
             return l;
         };
         auto member = [&](int t, int r, int rt) {
@@ -984,7 +993,7 @@ void Operator::ensure_levels(LevelSet &S) {
                     const double ex = ccen[3 * c] - pos_[3 * t], ey = ccen[3 * c + 1] - pos_[3 * t + 1],
                                  ez = ccen[3 * c + 2] - pos_[3 * t + 2];
                     if (std::sqrt(ex * ex + ey * ey + ez * ez) >= S.cell_theta * crho[c] + rad_[t]) {
-                        int lmin = kNumLevels;
+                        int lmin = nlev;
                         double member_pairs = 0.0;
                         for (int r = cell_r0[c]; r < cell_r1[c] && lmin >= 2; r++) {
                             const int sm = perm[r], lm = klass(t, sm);
@@ -1039,8 +1048,8 @@ void Operator::ensure_levels(LevelSet &S) {
         build_far_lists(S.mid, nsusp_, susp_sph_off_, pt_off_, pt_off_, mid, S.tpl, 0, kWantItems);
     }
     S.levels.clear();
-    std::vector<std::vector<int64_t>> level_off(kNumLevels);
-    for (int l = 0; l < kNumLevels; l++) {
+    std::vector<std::vector<int64_t>> level_off(nlev);
+    for (int l = 0; l < nlev; l++) {
         auto L = std::make_unique<FarLevel>();
         std::vector<int64_t> &off = level_off[l];
         off.assign(nsph_ + 1, 0);

@@ -142,10 +142,12 @@ struct CellAgg {
 
 // A multi-level far field: fine-grid sums for the pairs closer than the first
 // level's separation (mid), then one coarser grid per separation shell.
-constexpr int kNumLevels = 4;
+constexpr int kNumLevels = 4; // levels of the FP32 sets (source cells on their last two)
+constexpr int kMaxLevels = 6;
 struct LevelSet {
-    double alo[kNumLevels];  // separations (in units of R_s + R_t) where levels 0..3 start
-    int order[kNumLevels];   // grid order of each level (capped by the sphere's order)
+    int nlev = kNumLevels;   // levels in use
+    double alo[kMaxLevels];  // separations (in units of R_s + R_t) where levels 0..nlev-1 start
+    int order[kMaxLevels];   // grid order of each level (capped by the sphere's order)
     bool fp64 = false;       // FP64 pair sums (else FP32)
     int tpl = 1;             // targets per lane of its work lists
     std::vector<std::unique_ptr<FarLevel>> levels;
