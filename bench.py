@@ -15,10 +15,12 @@ the final single layer and the rigid-body averages.
   tests/golden/make_cfg5_fp64_reference.py in two GPU jobs), labelled
   "extrapolated"; the fixture's own two-job wall-clock is reported beside it.
 * Error: the mixed velocities against that FP64 solution.
-* roofline: the mixed form's dominant kernel, k_far_f32 (per apply seven
-  lists: mid range, four levels, two source-cell lists), over all its launches
-  in the timed solves (the coarse operator's and the FP32 operator's), from
-  HIP events the library records on its own stream around those launches.
+* roofline: the mixed form's pair-sum kernel with the most time per timed
+  solve, from HIP events the library records on its own stream around the
+  launches of each apply: k_far_f64_lvl (the FP64 residual apply: mid range
+  and six levels) or k_far_f32 (per apply seven lists: mid range, four
+  levels, two source-cell lists; the coarse operator's applies and the FP32
+  operator's); the other is reported under "mixed".
 * cpu_baseline: the oracle's full operator apply (far, self and near terms)
   on a bounded sample of target spheres, scaled to a full FP64 solve with the
   FP64 apply count (kind "port", all host threads), labelled "extrapolated".
@@ -111,6 +113,17 @@ def far_roofline(st, peak, kernel, launches_per_apply):
             "frac": ach / peak, "traffic": None, "avg_launch_ms": apply_ms / launches_per_apply,
             "launches_per_apply": launches_per_apply, "avg_apply_ms": apply_ms,
             "pairs_per_apply": st["pairs"], "flops_per_pair": FLOPS_TRAC, "applies_timed": st["launches"]}
+
+
+def dominant(a, b, st_a, st_b, nsolves):
+    """(the roofline object with more kernel time per solve, the other); each
+    gains ms_per_solve."""
+    for r, st in ((a, st_a), (b, st_b)):
+        if r is not None:
+            r["ms_per_solve"] = st["ms"] / max(1, nsolves)
+    if b is not None and (a is None or b["ms_per_solve"] > a["ms_per_solve"]):
+        return b, a
+    return a, b
 
 
 def cpu_baseline(c, applies, budget_s=8.0):
@@ -242,18 +255,23 @@ def main():
                    "setup_s": setup64, "two_job_solve_s": ref_meta["fp64_solve_s"],
                    "far_traction": far_roofline(st64, FP64_VALU_PEAK, "k_far_f64<traction>", 1)}
     out["mixed_vs_fp64"] = {"speedup": t_fp64 / (ms / 1000.0), "rel_err_V": eV, "rel_err_Omega": eW}
-    # k_far_f32 is the dominant kernel of the mixed solve: the coarse operator's
-    # lists (every Krylov iteration) and the FP32 operator's (its residuals)
+    # the mixed solve's pair-sum kernels: k_far_f32 (the coarse operator's lists
+    # every Krylov iteration, the FP32 operator's for its residuals) and
+    # k_far_f64_lvl (the FP64 residual apply); `roofline` is the one with more
+    # time per timed solve, the other goes under "mixed"
     both = {"ms": stc["ms"] + st32["ms"], "launches": stc["launches"] + st32["launches"],
             "pairs": (stc["pairs"] * stc["launches"] + st32["pairs"] * st32["launches"]) /
                      max(1, stc["launches"] + st32["launches"])}
-    out["roofline"] = far_roofline(both, FP32_VALU_PEAK, "k_far_f32<traction>: all FP32 pair-sum launches of a "
-                                   "solve (coarse operator: mid range, 4 levels, 2 cell lists per apply; FP32 "
-                                   "operator: likewise)", 7)
-    out["roofline"]["coarse"] = far_roofline(stc, FP32_VALU_PEAK, "k_far_f32<traction>, coarse operator", 7)
-    out["roofline"]["inner"] = far_roofline(st32, FP32_VALU_PEAK, "k_far_f32<traction>, FP32 operator", 7)
-    out["mixed"]["fp64_outer_far"] = far_roofline(st64ml, FP64_VALU_PEAK, "k_far_f64_lvl<traction>: the mid-range "
-                                                  "list and 6 level lists of one FP64 residual apply", 7)
+    f32 = far_roofline(both, FP32_VALU_PEAK, "k_far_f32<traction>: all FP32 pair-sum launches of a solve (coarse "
+                       "operator: mid range, 4 levels, 2 cell lists per apply; FP32 operator: likewise)", 7)
+    if f32 is not None:
+        f32["coarse"] = far_roofline(stc, FP32_VALU_PEAK, "k_far_f32<traction>, coarse operator", 7)
+        f32["inner"] = far_roofline(st32, FP32_VALU_PEAK, "k_far_f32<traction>, FP32 operator", 7)
+    f64 = far_roofline(st64ml, FP64_VALU_PEAK, "k_far_f64_lvl<traction>: the mid-range list and 6 level lists of "
+                       "one FP64 residual apply", 7)
+    out["roofline"], other = dominant(f32, f64, both, st64ml, cnt["steps"])
+    if other is not None:
+        out["mixed"]["fp32_far" if other is f32 else "fp64_outer_far"] = other
     if not args.no_cpu:
         log("CPU baseline (oracle, bounded sample)")
         out["cpu_baseline"] = cpu_baseline(c, fp64_applies)
